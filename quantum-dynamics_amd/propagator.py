@@ -1,0 +1,201 @@
+"""Host-side mirror of the reference's propagator entry points.
+
+``TimeEvolutionMex(data)`` and ``TimeEvolutionMexCUDA(data)`` take the same
+struct the MATLAB drivers build (``H3/main.m:122-140``): a dict with fields
+``r1, r2, theta, pot, psi, time, options, dump1, dump2, CRP`` carrying the
+same sub-fields (``common/TimeEvolutionMex.C:30-68``).  As in the reference
+they return nothing and mutate ``psi``, ``time['steps']`` and ``CRP['CRP']``
+in place.  Both run on the GPU through libqd_hip.so; the second selects the
+CUDA propagator's semantics (dump before the step, 0-based surface index,
+Zdotc sign; common-cuda/evolutionCUDA2.cu:487-552,898-994).
+
+``Propagator`` is the handle API (``qd_create`` / ``qd_propagate``): it keeps
+the flux accumulators across calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from .problem import Problem
+
+PRECISIONS = {"fp64": _abi.QD_PREC_FP64, "mixed": _abi.QD_PREC_MIXED}
+PIPELINES = {"fused": _abi.QD_PIPE_FUSED, "plain": _abi.QD_PIPE_PLAIN}
+VARIANTS = {"cpu": _abi.QD_VARIANT_CPU, "cuda": _abi.QD_VARIANT_CUDA}
+
+
+def make_options(precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
+                 device=0, profile=0, theta_split=0, use_graph=0):
+    return _abi.qd_options(PRECISIONS[precision], PIPELINES[pipeline], VARIANTS[variant], flux_block,
+                           crp_every, use_graph, device, profile, theta_split, 0)
+
+
+def _field(d, name, required=True):
+    if name not in d:
+        if required:
+            # the reference dereferences a missing field (matutils.h:23-31); here it is an error
+            raise KeyError("missing struct field %r" % name)
+        return None
+    return d[name]
+
+
+def _f64(a):
+    a = np.asarray(a)
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("arrays must be contiguous float64")
+    return a
+
+
+def problem_from_struct(data: dict) -> Problem:
+    """Read the reference struct into a Problem that shares psi and CRP.CRP."""
+    r1, r2, th = _field(data, "r1"), _field(data, "r2"), _field(data, "theta")
+    tm, crp = _field(data, "time"), _field(data, "CRP")
+    d1 = _field(data, "dump1", required=False) or {}
+    d2 = _field(data, "dump2", required=False) or {}
+    n = int(th["n"])
+    m = int(th["m"])
+    leg = np.asarray(th["legendre"], dtype=np.float64)
+    if leg.size != (m + 1) * n:
+        raise ValueError("theta.legendre must hold (m+1) x n values")
+    # legendre is (m+1) x n (MATLAB layout, l fastest); keep it as the (m+1, n) matrix
+    leg = leg.reshape((m + 1, n), order="F") if leg.ndim == 1 else leg.reshape((m + 1, n))
+    psi = data["psi"]
+    if psi.dtype != np.float64 or not psi.flags["C_CONTIGUOUS"] and not psi.flags["F_CONTIGUOUS"]:
+        raise TypeError("psi must be a contiguous float64 array (interleaved re/im)")
+    crp_out = crp["CRP"]
+    return Problem(
+        name="struct", r1=_f64(r1["r"]).reshape(-1), r2=_f64(r2["r"]).reshape(-1), dr1=float(r1["dr"]),
+        dr2=float(r2["dr"]), mass1=float(r1["mass"]), mass2=float(r2["mass"]),
+        x=_f64(th["x"]).reshape(-1), w=_f64(th["w"]).reshape(-1), m=m, legendre=np.ascontiguousarray(leg),
+        pot=np.ascontiguousarray(np.asarray(data["pot"], dtype=np.float64).reshape(-1, order="F")),
+        psi=psi.reshape(-1, order="A"), dt=float(tm["time_step"]), total_steps=int(tm["total_steps"]),
+        steps=int(tm["steps"]),
+        dump1=None if d1.get("dump") is None else _f64(d1["dump"]).reshape(-1),
+        dump2=None if d2.get("dump") is None else _f64(d2["dump"]).reshape(-1),
+        n_dividing_surface=int(crp["n_dividing_surface"]), n_gradient_points=int(crp["n_gradient_points"]),
+        energies=_f64(crp["energies"]).reshape(-1), eta_sq=_f64(crp["eta_sq"]).reshape(-1),
+        crp=crp_out.reshape(-1), calculate_crp=int(crp.get("calculate_CRP", 1)))
+
+
+def _run_struct(data: dict, variant: str, **opts):
+    lib = _abi.load_library()
+    p = problem_from_struct(data)
+    st = _abi.ProblemStruct(p)
+    o = make_options(variant=variant, **opts)
+    _abi.check(lib.qd_time_evolution(C.byref(st.s), C.byref(o)))
+    data["time"]["steps"] = int(st.s.time.steps)
+
+
+def TimeEvolutionMex(data: dict, **opts) -> None:
+    """GPU drop-in for the CPU propagator TimeEvolutionMex (common/TimeEvolutionMex.C:18-78)."""
+    _run_struct(data, "cpu", **opts)
+
+
+def TimeEvolutionMexCUDA(data: dict, **opts) -> None:
+    """GPU drop-in for TimeEvolutionMexCUDA (common-cuda/TimeEvolutionMexCUDA.C:20-83)."""
+    _run_struct(data, "cuda", **opts)
+
+
+def problem_to_struct(p: Problem) -> dict:
+    """The MATLAB-style struct for a Problem (fields as H3/main.m builds them)."""
+    return {
+        "r1": {"n": p.n1, "r": p.r1, "dr": p.dr1, "mass": p.mass1},
+        "r2": {"n": p.n2, "r": p.r2, "dr": p.dr2, "mass": p.mass2},
+        "theta": {"n": p.ntheta, "m": p.m, "x": p.x, "w": p.w, "legendre": p.legendre},
+        "pot": p.pot,
+        "psi": p.psi,
+        "time": {"total_steps": p.total_steps, "time_step": p.dt, "steps": p.steps},
+        "options": {},
+        "dump1": {"dump": p.dump1},
+        "dump2": {"dump": p.dump2},
+        "CRP": {"n_dividing_surface": p.n_dividing_surface, "n_gradient_points": p.n_gradient_points,
+                "n_energies": p.energies.size, "calculate_CRP": p.calculate_crp,
+                "energies": p.energies, "eta_sq": p.eta_sq, "CRP": p.crp},
+    }
+
+
+class Propagator:
+    """Handle API over qd_create / qd_propagate / qd_get_* / qd_destroy."""
+
+    def __init__(self, problem: Problem, precision="fp64", pipeline="fused", variant="cpu",
+                 flux_block=0, crp_every=0, device=0, profile=0, theta_split=0):
+        self.lib = _abi.load_library()
+        self.problem = problem
+        self._st = _abi.ProblemStruct(problem)
+        self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile,
+                                 theta_split)
+        h = C.c_void_p()
+        _abi.check(self.lib.qd_create(C.byref(self._st.s), C.byref(self._opt), C.byref(h)))
+        self.h = h
+        self._cb = None
+
+    def propagate(self, n_steps: int):
+        _abi.check(self.lib.qd_propagate(self.h, int(n_steps)))
+
+    def set_callback(self, fn):
+        """fn(steps, crp_or_None) -> bool (True stops)."""
+        ne = self.problem.energies.size
+
+        def tramp(steps, crp_ptr, user):
+            crp = None
+            if crp_ptr:
+                crp = np.ctypeslib.as_array(crp_ptr, shape=(ne,)).copy()
+            return 1 if fn(int(steps), crp) else 0
+
+        self._cb = _abi.QD_STEP_CB(tramp)
+        _abi.check(self.lib.qd_set_callback(self.h, self._cb, None))
+
+    def psi(self) -> np.ndarray:
+        out = np.empty(2 * self.problem.grid_points)
+        _abi.check(self.lib.qd_get_psi(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def crp(self) -> np.ndarray:
+        out = np.empty(self.problem.energies.size)
+        _abi.check(self.lib.qd_get_crp(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def norm(self) -> float:
+        v = C.c_double()
+        _abi.check(self.lib.qd_get_norm(self.h, C.byref(v)))
+        return v.value
+
+    def steps(self) -> int:
+        v = C.c_int32()
+        _abi.check(self.lib.qd_get_steps(self.h, C.byref(v)))
+        return v.value
+
+    def timing(self, which: int) -> float:
+        v = C.c_double()
+        _abi.check(self.lib.qd_get_timing(self.h, int(which), C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            self.lib.qd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run(problem: Problem, steps: Optional[int] = None, **opts):
+    """Propagate a copy of problem.psi for `steps` steps; returns (psi, crp)."""
+    p = problem.copy()
+    if steps is not None:
+        p.total_steps = steps
+    with Propagator(p, **opts) as prop:
+        prop.propagate(p.total_steps)
+        return prop.psi(), (prop.crp() if p.calculate_crp else None)

@@ -1,0 +1,25 @@
+import importlib
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libqd_hip.so)")
+
+
+@pytest.fixture(scope="session")
+def qd():
+    return importlib.import_module("quantum-dynamics_amd")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import binding
+    return binding

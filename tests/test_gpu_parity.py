@@ -1,0 +1,71 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: FP64 psi and P(E) within 1e-8 relative of the oracle
+(|dP_e| <= 1e-8 * max(max_e |P|, 1e-2)); the mixed path within 1e-4 of the
+FP64 path on the same grid.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_psi(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def crp_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-2))
+
+
+@pytest.fixture(scope="module")
+def cfg0(qd):
+    return qd.problem.config(0)
+
+
+@pytest.mark.parametrize("pipeline", ["fused", "plain"])
+@pytest.mark.parametrize("variant", ["cpu", "cuda"])
+def test_fp64_short_matches_oracle(qd, oracle, cfg0, pipeline, variant):
+    steps = 30
+    print("case: cfg0 fp64", pipeline, variant, steps, flush=True)
+    psi_g, crp_g = qd.run(cfg0, steps=steps, precision="fp64", pipeline=pipeline, variant=variant)
+    psi_o, crp_o, _, _ = oracle.time_evolution(cfg0, total_steps=steps, variant=1 if variant == "cuda" else 0)
+    assert rel_psi(psi_g, psi_o) < 1e-10
+    assert crp_err(crp_g, crp_o) < 1e-8
+
+
+def test_fp64_full_cfg0_matches_oracle(qd, oracle, cfg0):
+    print("case: cfg0 fp64 full 2000 steps", flush=True)
+    psi_g, crp_g = qd.run(cfg0, precision="fp64")
+    psi_o, crp_o, _, _ = oracle.time_evolution(cfg0)
+    print("dpsi", rel_psi(psi_g, psi_o), "dP", crp_err(crp_g, crp_o))
+    assert crp_err(crp_g, crp_o) < 1e-8
+
+
+@pytest.mark.parametrize("split", [2, 3])
+def test_mixed_short_close_to_fp64(qd, cfg0, split):
+    print("case: cfg0 mixed split", split, flush=True)
+    psi_d, crp_d = qd.run(cfg0, steps=200, precision="fp64")
+    psi_m, crp_m = qd.run(cfg0, steps=200, precision="mixed", theta_split=split)
+    print("dpsi", rel_psi(psi_m, psi_d), "dP", np.max(np.abs(crp_m - crp_d)))
+    assert rel_psi(psi_m, psi_d) < 1e-3
+    assert np.max(np.abs(crp_m - crp_d)) < 1e-3
+
+
+def test_handle_resume_and_norm(qd, cfg0):
+    p = cfg0.copy()
+    p.dump1 = None
+    p.dump2 = None
+    with qd.Propagator(p, precision="fp64") as prop:
+        n0 = prop.norm()
+        prop.propagate(10)
+        prop.propagate(15)
+        n1 = prop.norm()
+        assert prop.steps() == 25
+    assert abs(n1 - n0) < 1e-12
+    psi_a, crp_a = qd.run(p, steps=25, precision="fp64")
+    with qd.Propagator(p, precision="fp64") as prop:
+        prop.propagate(10)
+        prop.propagate(15)
+        assert rel_psi(prop.psi(), psi_a) < 1e-14
+        assert crp_err(prop.crp(), crp_a) < 1e-12
