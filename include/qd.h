@@ -151,6 +151,51 @@ int qd_get_steps(qd_handle *h, int32_t *steps_out);
 int qd_get_timing(qd_handle *h, int32_t which, double *ms_out);
 int qd_destroy(qd_handle *h);
 
+/* Replace the handle's psi (same layout as qd_get_psi); time.steps and the
+ * flux accumulators are kept. */
+int qd_set_psi(qd_handle *h, const double *psi_in);
+
+/* ---- J > 0: body-fixed (J, K) blocks ----------------------------------
+ * No counterpart in the reference, which propagates J = 0 only (the rotational
+ * term of common/timeEvol.C:487 is l(l+1)(1/2mu1R^2 + 1/2mu2r^2)).  Block
+ * (J, K) holds n_K = jmax - K + 1 theta points: the DVR of the normalised
+ * associated Legendre functions P_j^K, j = K..jmax (Gauss-Legendre for K = 0).
+ * The K blocks of one J are Coriolis coupled through (J - j)^2 in the
+ * parity-adapted body-fixed basis.  A J = 0 block uses the reference's own
+ * Legendre matrices and reproduces qd_create on the same grid. */
+typedef struct qd_jk_block {
+  int32_t J, K;           /* 0 <= K <= min(J, jmax); every J needs K = 0..min(J, jmax) */
+  const double *pot;      /* V on the block's grid, n1*n2*n_K (blocks of one K share it) */
+  const double *psi;      /* initial psi, 2*n1*n2*n_K interleaved */
+} qd_jk_block;
+
+typedef struct qd_jk_problem {
+  qd_problem base;        /* r1, r2, time, dump1, dump2, crp; theta, pot and psi unused */
+  int32_t jmax;           /* highest j of the rotational basis */
+  int32_t coriolis;       /* 1: Coriolis-coupled; 0: centrifugal-sudden (K blocks uncoupled) */
+  int32_t n_blocks;
+  int32_t reserved;
+  const qd_jk_block *blocks;
+} qd_jk_problem;
+
+/* theta points x (ascending cos theta) and weights w of block K (n_K values each). */
+int qd_jk_theta_grid(int32_t K, int32_t jmax, double *x, double *w);
+/* Build a handle over all blocks; qd_propagate advances them together,
+ * qd_get_psi returns the blocks back to back in the given order, qd_get_crp the
+ * sum over blocks, qd_get_crp_blocks P(E) per block (n_blocks x n_energies). */
+int qd_create_jk(const qd_jk_problem *problem, const qd_options *options, qd_handle **out);
+int qd_get_crp_blocks(qd_handle *h, double *crp_out);
+
+/* ---- test hooks: one stage of the step on the handle's current arrays ----
+ * stage 0: theta forward (psi -> FBR array, no V phase); 1: R pass with phases,
+ * mixing and kinetic factor; 2: r pass with phases and kinetic factor;
+ * 3: theta backward (FBR array -> psi, no dump); 4: forward FFT along R only;
+ * 5: forward FFT along r only; 6: inverse FFT along R only; 7: inverse FFT
+ * along r only.  The FBR array has n1*n2*(FBR columns) complex values. */
+int qd_debug_stage(qd_handle *h, int32_t stage);
+int qd_debug_get_fbr(qd_handle *h, double *out);
+int qd_debug_set_fbr(qd_handle *h, const double *in);
+
 /* Last error message of the calling thread (empty string if none). */
 const char *qd_last_error(void);
 

@@ -51,6 +51,15 @@ class qd_options(C.Structure):
                 ("theta_split", C.c_int32), ("reserved", C.c_int32)]
 
 
+class qd_jk_block(C.Structure):
+    _fields_ = [("J", C.c_int32), ("K", C.c_int32), ("pot", _dp), ("psi", _dp)]
+
+
+class qd_jk_problem(C.Structure):
+    _fields_ = [("base", qd_problem), ("jmax", C.c_int32), ("coriolis", C.c_int32),
+                ("n_blocks", C.c_int32), ("reserved", C.c_int32), ("blocks", C.POINTER(qd_jk_block))]
+
+
 QD_STEP_CB = C.CFUNCTYPE(C.c_int, C.c_int32, _dp, C.c_void_p)
 
 
@@ -118,9 +127,18 @@ def load_library(path: str = LIB_PATH):
     lib.qd_get_steps.argtypes = [C.c_void_p, P(C.c_int32)]
     lib.qd_get_timing.argtypes = [C.c_void_p, C.c_int32, _dp]
     lib.qd_destroy.argtypes = [C.c_void_p]
+    lib.qd_set_psi.argtypes = [C.c_void_p, _dp]
+    lib.qd_create_jk.argtypes = [P(qd_jk_problem), P(qd_options), P(C.c_void_p)]
+    lib.qd_jk_theta_grid.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
+    lib.qd_get_crp_blocks.argtypes = [C.c_void_p, _dp]
+    lib.qd_debug_stage.argtypes = [C.c_void_p, C.c_int32]
+    lib.qd_debug_get_fbr.argtypes = [C.c_void_p, _dp]
+    lib.qd_debug_set_fbr.argtypes = [C.c_void_p, _dp]
     lib.qd_last_error.restype = C.c_char_p
     for f in ("qd_time_evolution", "qd_create", "qd_propagate", "qd_set_callback", "qd_get_psi",
-              "qd_get_crp", "qd_get_norm", "qd_get_steps", "qd_get_timing", "qd_destroy"):
+              "qd_get_crp", "qd_get_norm", "qd_get_steps", "qd_get_timing", "qd_destroy", "qd_set_psi",
+              "qd_create_jk", "qd_jk_theta_grid", "qd_get_crp_blocks", "qd_debug_stage", "qd_debug_get_fbr",
+              "qd_debug_set_fbr"):
         getattr(lib, f).restype = C.c_int
     _lib = lib
     return lib

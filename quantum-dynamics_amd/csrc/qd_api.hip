@@ -4,12 +4,16 @@
 // handles on different threads or devices are independent (the reference's
 // CUDA path keeps module-global __constant__ state, evolutionCUDA2.cu:19-24).
 //
+// A handle propagates a batch of (J, K) blocks on shared R, r grids.  The
+// reference's J = 0 problem (qd_create / qd_time_evolution) is a batch of one
+// block; qd_create_jk builds the body-fixed K blocks of several J.
+//
 // Per step (fused pipeline, CPU-propagator semantics, timeEvol.C:657-684):
-//   k_theta_*<V prologue>       psi -> lpsi              (DVR -> FBR)
-//   k_fft_lines<R>              rot/2, FFT_R, T1, IFFT_R
-//   k_fft_lines<r>              FFT_r, T2, IFFT_r, rot/2
-//   k_theta_*<dump epilogue>    lpsi -> psi              (FBR -> DVR)
-//   k_flux_extract              psi_s, dpsi_s -> time-block slot
+//   k_theta_*<V prologue>     psi -> lpsi                   (DVR -> FBR)
+//   k_rpass                   K->l, D_R/2, FFT_R, T1, IFFT_R, D_R/2, l->K
+//   k_rline                   D_r/2, FFT_r, T2, IFFT_r, D_r/2
+//   k_theta_*<dump epilogue>  lpsi -> psi                   (FBR -> DVR)
+//   k_flux_extract            psi_s, dpsi_s -> time-block slot
 // and every flux_block steps k_flux_accumulate folds the block into the
 // energy-resolved accumulators (CRPAux.F:4-31).  P(E) is evaluated on demand.
 #include <hip/hip_runtime.h>
@@ -18,11 +22,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/qd.h"
 #include "qd_kernels.hpp"
+#include "qd_tables.hpp"
 
 using namespace qd;
 
@@ -41,21 +47,19 @@ int fail(int code, const std::string &msg) {
     if (e_ != hipSuccess)                                                                    \
       return fail(QD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));         \
   } while (0)
+#define QD_RC(x)                  \
+  do {                            \
+    int rc_ = (x);                \
+    if (rc_ != QD_OK) return rc_; \
+  } while (0)
 
 enum { TK_STEP = 0, TK_THETA = 1, TK_FFTR1 = 2, TK_FFTR2 = 3, TK_FLUX = 4, TK_OTHER = 5, TK_N = 6 };
-
-struct FftCfg {
-  FftPlanArg plan;
-  int lpb = 1;
-  int nblocks = 0;
-  size_t lds = 0;
-};
 
 // bf16 round-to-nearest-even of a float, as the device conversion does
 uint16_t bf16_bits(float f) {
   uint32_t u;
   memcpy(&u, &f, 4);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
   const uint32_t lsb = (u >> 16) & 1u;
   u += 0x7fffu + lsb;
   return (uint16_t)(u >> 16);
@@ -85,6 +89,51 @@ bool factor_plan(int n, FftPlanArg &p) {
   return true;
 }
 
+int max_butterflies(const FftPlanArg &p) {
+  int m = 0;
+  for (int i = 0; i < p.npass; i++) m = std::max(m, p.n / p.radix[i]);
+  return m;
+}
+
+int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
+// Matrices of one theta basis: Wf[k][j] (nd x nf), Pb[j][k] (nf x nd), DVR weights.
+struct MatSet {
+  int nd = 0, nf = 0;
+  std::vector<double> Wf, Pb, w;
+};
+
+// One block of the batch as the builder sees it.
+struct HostBlock {
+  int J = 0, K = 0;
+  int mat = 0;                  // theta matrix set
+  int vset = 0;                 // potential set (blocks of one K share it)
+  const double *psi = nullptr;  // initial psi (interleaved), n12 * nd complex
+};
+
+// Mixing unit group of the R pass: FBR columns (eigen slot m <-> column of K = m) and U.
+struct Group {
+  std::vector<int> cols;
+  int u0 = -1;
+};
+
+// Everything needed to build a handle, from either entry point.
+struct BatchSpec {
+  const qd_problem *base = nullptr;       // grids, time, dumps, flux parameters
+  std::vector<HostBlock> blocks;
+  std::vector<MatSet> mats;
+  std::vector<const double *> vsets;      // potential of each set, n12 * nd
+  std::vector<long long> fcol_lamval;     // lambda = (J - j)^2 eigenvalue per FBR column (l basis)
+  std::vector<int> fcol_j;                // j per FBR column
+  std::vector<Group> groups;
+  std::vector<double> umat;               // concatenated U matrices (row K, column m)
+  int jmax = 0;
+};
+
 }  // namespace
 
 struct qd_handle {
@@ -97,36 +146,52 @@ struct qd_handle {
   int crp_every = 0;
   int profile = 0;
   int nsplit = 3;
-  int n1 = 0, n2 = 0, nth = 0, M = 0, ne = 0;
+  int n1 = 0, n2 = 0, ne = 0;
+  int nb = 0, ndcol = 0, nfcol = 0;
   size_t n12 = 0, npt = 0, nlp = 0, nx = 0;
   double dt = 0, dr1 = 0, dr2 = 0, mu2 = 0;
   bool has_dump = false;
   bool do_crp = false;
+  bool has_mix = false;
   int surf = 0;  // 0-based surface row
   StencilArg stencil{};
+  std::vector<BlockDesc> hblocks;
   // state
-  int32_t steps = 0;          // time.steps
+  int32_t steps = 0;
   bool pending_prephase = false;
-  long long acc_steps = 0;    // steps folded into fai/dfai
-  int slot = 0;               // filled slots of the current time block
+  long long acc_steps = 0;
+  int slot = 0;
   qd_step_cb cb = nullptr;
   void *cb_user = nullptr;
   hipStream_t stream = nullptr;
-  // device buffers (FP64 path uses *64, mixed uses *32)
+  // device buffers
   double2 *psi64 = nullptr, *lpsi64 = nullptr;
   float2 *psi32 = nullptr, *lpsi32 = nullptr;
-  double *V = nullptr;              // FP64 potential
-  float2 *expV32 = nullptr;         // mixed: exp(-i V dt) rounded to FP32
+  double *V = nullptr;
+  float2 *expV32 = nullptr;
+  long long *dcol_vidx = nullptr;
+  double *dcol_w = nullptr;
+  int *fcol_lam = nullptr, *fcol_j = nullptr;
+  BlockDesc *d_blocks = nullptr;
   double *d1 = nullptr, *d2 = nullptr;
   float *d1f = nullptr, *d2f = nullptr;
-  void *rotR = nullptr, *rotr = nullptr, *kinR = nullptr, *kinr = nullptr, *twR = nullptr, *twr = nullptr;
-  double *Wf64 = nullptr, *Wb64 = nullptr;   // packed [Kpad][Npad]
-  __bf16 *Wf16 = nullptr, *Wb16 = nullptr;   // split [NS][Npad][Kpad]
-  int Wf_Npad = 0, Wb_Npad = 0, Wf_Kpad = 0, Wb_Kpad = 0;
+  void *dR = nullptr, *dr = nullptr, *kinR = nullptr, *kinr = nullptr, *twR = nullptr, *twr = nullptr;
+  void *Wf = nullptr, *Wb = nullptr;  // packed FP64 [mat][Kpad][Npad] or bf16 [mat][3][Npad][Kpad]
+  long long mat_stride = 0;
+  int mat_Kpad = 0, mat_Npad = 0, ncb = 8;
+  int col_tiles = 1;
+  int *bundle_u0 = nullptr;
+  RUnit *units = nullptr;
+  int *unit_cols = nullptr;
+  void *umat = nullptr;
+  int nbundle = 0, rp_tpl = 16;
+  size_t rp_lds = 0;
+  FftPlanArg planR{}, planr{};
+  int rl_lpb = 16, rl_blocks = 0;
+  size_t rl_lds = 0;
   double2 *bpsi = nullptr, *bdpsi = nullptr, *fai = nullptr, *dfai = nullptr;
-  double *energies = nullptr, *eta_sq = nullptr, *w = nullptr, *crp = nullptr, *partial = nullptr;
+  double *energies = nullptr, *eta_sq = nullptr, *crp = nullptr, *partial = nullptr;
   int npartial = 256;
-  FftCfg fR, fr;
   // timing
   hipEvent_t ev[2] = {};
   std::vector<hipEvent_t> marks;
@@ -135,121 +200,21 @@ struct qd_handle {
   double t_ms[TK_N] = {};
   long long t_n[TK_N] = {};
   double last_step_ms = 0;
-  std::vector<double> host_crp;
+  std::vector<double> host_crp;  // [nb][ne]
 };
 
 namespace {
 
 template <typename T>
 int upload(T **dst, const T *src, size_t n) {
-  QD_HIP(hipMalloc((void **)dst, n * sizeof(T)));
-  QD_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-  return QD_OK;
-}
-
-int fft_config(int n, int axis, size_t elem, int nlines, FftCfg &c) {
-  if (!factor_plan(n, c.plan)) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n) + " has a prime factor > 5");
-  int maxb = 0;
-  for (int p = 0; p < c.plan.npass; p++) maxb = std::max(maxb, n / c.plan.radix[p]);
-  int lpb;
-  if (axis == 0) {
-    lpb = 1;
-    while (lpb * 2 <= 16 && lpb * 2 * n <= 1024) lpb *= 2;
-  } else {
-    lpb = 16;
-  }
-  while (lpb > 1 && ((256 / lpb) * FFT_BPT < maxb || (size_t)lpb * n * elem > 65536)) lpb /= 2;
-  if ((256 / lpb) * FFT_BPT < maxb || (size_t)lpb * n * elem > 65536)
-    return fail(QD_ERR_ARG, "FFT length " + std::to_string(n) + " too large");
-  c.lpb = lpb;
-  c.nblocks = (nlines + lpb - 1) / lpb;
-  c.lds = (size_t)lpb * n * elem;
-  return QD_OK;
-}
-
-template <typename T>
-int launch_fft(qd_handle *h, int axis, int flags, typename cx_of<T>::type *data) {
-  FftArgs<T> a;
-  a.data = data;
-  const FftCfg &c = axis == 0 ? h->fR : h->fr;
-  a.tw = (const typename cx_of<T>::type *)(axis == 0 ? h->twR : h->twr);
-  a.kin = (const typename cx_of<T>::type *)(axis == 0 ? h->kinR : h->kinr);
-  a.rotR = (const typename cx_of<T>::type *)h->rotR;
-  a.rotr = (const typename cx_of<T>::type *)h->rotr;
-  a.n1 = h->n1;
-  a.n2 = h->n2;
-  a.m = h->M;
-  a.lpb = c.lpb;
-  a.nlines = axis == 0 ? h->n2 * h->M : h->n1 * h->M;
-  a.flags = flags;
-  a.plan = c.plan;
-  if (axis == 0)
-    hipLaunchKernelGGL((k_fft_lines<T, 0>), dim3(c.nblocks), dim3(256), c.lds, h->stream, a);
-  else
-    hipLaunchKernelGGL((k_fft_lines<T, 1>), dim3(c.nblocks), dim3(256), c.lds, h->stream, a);
-  QD_HIP(hipGetLastError());
-  return QD_OK;
-}
-
-// theta transform: fwd (psi -> lpsi, K = nth, N = M) or bwd (lpsi -> psi, K = M, N = nth)
-int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
-  const int R = (int)h->n12;
-  const int K = fwd ? h->nth : h->M;
-  const int N = fwd ? h->M : h->nth;
-  dim3 grid((R + TG_ROWS - 1) / TG_ROWS, (N + TG_COLS - 1) / TG_COLS);
-  if (h->prec == QD_PREC_FP64) {
-    const double2 *A = fwd ? h->psi64 : h->lpsi64;
-    double2 *Cm = fwd ? h->lpsi64 : h->psi64;
-    const double *B = fwd ? h->Wf64 : h->Wb64;
-    const int Npad = fwd ? h->Wf_Npad : h->Wb_Npad;
-#define QD_T64(P, E) \
-  hipLaunchKernelGGL((k_theta_f64<P, E>), grid, dim3(256), 0, h->stream, A, Cm, B, R, K, N, Npad, h->V, h->dt, h->d1, h->d2, h->n1)
-    if (pro == 0 && epi == 0) QD_T64(0, 0);
-    else if (pro == 1 && epi == 0) QD_T64(1, 0);
-    else if (pro == 2 && epi == 0) QD_T64(2, 0);
-    else if (pro == 0 && epi == 1) QD_T64(0, 1);
-    else return fail(QD_ERR_ARG, "bad theta variant");
-#undef QD_T64
-  } else {
-    const float2 *A = fwd ? h->psi32 : h->lpsi32;
-    float2 *Cm = fwd ? h->lpsi32 : h->psi32;
-    const __bf16 *B = fwd ? h->Wf16 : h->Wb16;
-    const int Npad = fwd ? h->Wf_Npad : h->Wb_Npad;
-    const int Kpad = fwd ? h->Wf_Kpad : h->Wb_Kpad;
-#define QD_T16(P, E)                                                                                     \
-  do {                                                                                                   \
-    if (h->nsplit == 2) {                                                                                \
-      constexpr int NSPL = 2;                                                                            \
-      hipLaunchKernelGGL((k_theta_bf16<NSPL, P, E>), grid, dim3(256), 0, h->stream, A, Cm, B, R, K, N, Npad, \
-                         Kpad, h->expV32, h->d1f, h->d2f, h->n1);                                         \
-    } else {                                                                                             \
-      constexpr int NSPL = 3;                                                                            \
-      hipLaunchKernelGGL((k_theta_bf16<NSPL, P, E>), grid, dim3(256), 0, h->stream, A, Cm, B, R, K, N, Npad, \
-                         Kpad, h->expV32, h->d1f, h->d2f, h->n1);                                         \
-    }                                                                                                    \
-  } while (0)
-    if (pro == 0 && epi == 0) QD_T16(0, 0);
-    else if (pro == 1 && epi == 0) QD_T16(1, 0);
-    else if (pro == 2 && epi == 0) QD_T16(2, 0);
-    else if (pro == 0 && epi == 1) QD_T16(0, 1);
-    else return fail(QD_ERR_ARG, "bad theta variant");
-#undef QD_T16
-  }
-  QD_HIP(hipGetLastError());
+  QD_HIP(hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) QD_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
   return QD_OK;
 }
 
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-#define QD_RC0(x)                 \
-  do {                            \
-    int rc0_ = (x);               \
-    if (rc0_ != QD_OK) return rc0_; \
-  } while (0)
-
-// Profiling: a chain of events, one before every kernel, read back in
-// batches so that the host never waits inside a step.  The segment that
-// starts at event i is charged to the kernel class recorded with it.
+// ---- profiling: a chain of events, one before every kernel, drained in batches ----
 int drain_marks(qd_handle *h) {
   const int n = h->nmarks;
   if (n < 2) return QD_OK;
@@ -263,7 +228,6 @@ int drain_marks(qd_handle *h) {
       h->t_n[c] += 1;
     }
   }
-  // the newest event starts the next chain
   std::swap(h->marks[0], h->marks[n - 1]);
   h->mark_cls[0] = h->mark_cls[n - 1];
   h->nmarks = 1;
@@ -274,25 +238,115 @@ int mark(qd_handle *h, int cls) {
   QD_HIP(hipEventRecord(h->marks[h->nmarks], h->stream));
   h->mark_cls[h->nmarks] = cls;
   h->nmarks++;
-  if (h->nmarks == (int)h->marks.size()) QD_RC0(drain_marks(h));
+  if (h->nmarks == (int)h->marks.size()) QD_RC(drain_marks(h));
   return QD_OK;
 }
-int tick(qd_handle *h, int k, bool) { return mark(h, k); }
-int tock(qd_handle *, int) { return QD_OK; }
-#define QD_RC(x)               \
-  do {                         \
-    int rc_ = (x);             \
-    if (rc_ != QD_OK) return rc_; \
+
+// ---- launches ----
+template <typename T>
+int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
+  typedef typename cx_of<T>::type C;
+  RPassArgs<T> a;
+  a.data = data;
+  a.tw = (const C *)h->twR;
+  a.kin = (const C *)h->kinR;
+  a.dR = (const C *)h->dR;
+  a.fcol_lam = h->fcol_lam;
+  a.bundle_u0 = h->bundle_u0;
+  a.units = h->units;
+  a.unit_cols = h->unit_cols;
+  a.umat = (const T *)h->umat;
+  a.n1 = h->n1;
+  a.n2 = h->n2;
+  a.tpl = h->rp_tpl;
+  a.flags = flags;
+  if (h->has_mix && (flags & FFT_PHASE)) a.flags |= FFT_MIX;
+  a.plan = h->planR;
+  hipLaunchKernelGGL((k_rpass<T>), dim3(h->nbundle), dim3(RP_THREADS), h->rp_lds, h->stream, a);
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+template <typename T>
+int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
+  typedef typename cx_of<T>::type C;
+  RLineArgs<T> a;
+  a.data = data;
+  a.tw = (const C *)h->twr;
+  a.kin = (const C *)h->kinr;
+  a.dr = (const C *)h->dr;
+  a.fcol_j = h->fcol_j;
+  a.n1 = h->n1;
+  a.n2 = h->n2;
+  a.lpb = h->rl_lpb;
+  a.nlines = h->n1 * h->nfcol;
+  a.flags = flags;
+  a.plan = h->planr;
+  hipLaunchKernelGGL((k_rline<T>), dim3(h->rl_blocks), dim3(256), h->rl_lds, h->stream, a);
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
+  ThetaArgs P;
+  P.A = h->prec == QD_PREC_FP64 ? (const void *)(fwd ? h->psi64 : h->lpsi64)
+                                : (const void *)(fwd ? h->psi32 : h->lpsi32);
+  P.C = h->prec == QD_PREC_FP64 ? (void *)(fwd ? h->lpsi64 : h->psi64) : (void *)(fwd ? h->lpsi32 : h->psi32);
+  P.B = fwd ? h->Wf : h->Wb;
+  P.mat_stride = h->mat_stride;
+  P.Kpad = h->mat_Kpad;
+  P.Npad = h->mat_Npad;
+  P.blocks = h->d_blocks;
+  P.fwd = fwd ? 1 : 0;
+  P.R = (int)h->n12;
+  P.V = h->V;
+  P.expV = h->expV32;
+  P.dcol_vidx = h->dcol_vidx;
+  P.dt = h->dt;
+  P.d1 = h->d1;
+  P.d2 = h->d2;
+  P.d1f = h->d1f;
+  P.d2f = h->d2f;
+  P.n1 = h->n1;
+  const int rows = (int)((h->n12 + TG_ROWS - 1) / TG_ROWS);
+  dim3 grid(rows, h->col_tiles, h->nb);
+  if (h->prec == QD_PREC_FP64) {
+#define QD_T64(PR, EP) hipLaunchKernelGGL((k_theta_f64<PR, EP>), grid, dim3(256), 0, h->stream, P)
+    if (pro == 0 && epi == 0) QD_T64(0, 0);
+    else if (pro == 1 && epi == 0) QD_T64(1, 0);
+    else if (pro == 2 && epi == 0) QD_T64(2, 0);
+    else if (pro == 0 && epi == 1) QD_T64(0, 1);
+    else return fail(QD_ERR_ARG, "bad theta variant");
+#undef QD_T64
+  } else {
+#define QD_T16(NS_, NCB_, PR, EP) \
+  hipLaunchKernelGGL((k_theta_bf16<NS_, NCB_, PR, EP>), grid, dim3(256), 0, h->stream, P)
+#define QD_T16_SEL(PR, EP)                                   \
+  do {                                                       \
+    if (h->nsplit == 2 && h->ncb == 8) QD_T16(2, 8, PR, EP); \
+    else if (h->nsplit == 2) QD_T16(2, 4, PR, EP);           \
+    else if (h->ncb == 8) QD_T16(3, 8, PR, EP);              \
+    else QD_T16(3, 4, PR, EP);                               \
   } while (0)
+    if (pro == 0 && epi == 0) QD_T16_SEL(0, 0);
+    else if (pro == 1 && epi == 0) QD_T16_SEL(1, 0);
+    else if (pro == 2 && epi == 0) QD_T16_SEL(2, 0);
+    else if (pro == 0 && epi == 1) QD_T16_SEL(0, 1);
+    else return fail(QD_ERR_ARG, "bad theta variant");
+#undef QD_T16_SEL
+#undef QD_T16
+  }
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
 
 int flush_flux(qd_handle *h) {
   if (!h->do_crp || h->slot == 0) return QD_OK;
-  QD_RC(tick(h, TK_FLUX, true));
+  QD_RC(mark(h, TK_FLUX));
   dim3 grid((unsigned)((h->nx + FX_X - 1) / FX_X), (unsigned)((h->ne + FX_E - 1) / FX_E));
   hipLaunchKernelGGL(k_flux_accumulate, grid, dim3(256), 0, h->stream, h->bpsi, h->bdpsi, h->fai, h->dfai,
                      h->energies, (int)h->nx, h->ne, h->slot, h->acc_steps, h->dt);
   QD_HIP(hipGetLastError());
-  QD_RC(tock(h, TK_FLUX));
   h->acc_steps += h->slot;
   h->slot = 0;
   return QD_OK;
@@ -301,17 +355,17 @@ int flush_flux(qd_handle *h) {
 int compute_crp(qd_handle *h) {
   QD_RC(flush_flux(h));
   const double sign = h->variant == QD_VARIANT_CUDA ? -1.0 : 1.0;
-  hipLaunchKernelGGL(k_crp, dim3(h->ne), dim3(256), 0, h->stream, h->fai, h->dfai, h->w, h->eta_sq, h->crp, h->n1,
-                     h->nth, h->dr1 / h->mu2, sign);
+  hipLaunchKernelGGL(k_crp, dim3(h->ne, h->nb), dim3(256), 0, h->stream, h->fai, h->dfai, h->dcol_w, h->eta_sq,
+                     h->d_blocks, h->crp, h->n1, (int)h->nx, h->ne, h->dr1 / h->mu2, sign);
   QD_HIP(hipGetLastError());
-  h->host_crp.resize(h->ne);
-  QD_HIP(hipMemcpyAsync(h->host_crp.data(), h->crp, sizeof(double) * h->ne, hipMemcpyDeviceToHost, h->stream));
+  h->host_crp.resize((size_t)h->nb * h->ne);
+  QD_HIP(hipMemcpyAsync(h->host_crp.data(), h->crp, sizeof(double) * h->host_crp.size(), hipMemcpyDeviceToHost,
+                        h->stream));
   QD_HIP(hipStreamSynchronize(h->stream));
   return QD_OK;
 }
 
 int apply_prephase(qd_handle *h) {
-  // runs on FP64 psi; the mixed path keeps an FP64 copy until the first step
   double2 *p64 = h->psi64;
   bool tmp = false;
   if (h->prec == QD_PREC_MIXED) {
@@ -320,9 +374,11 @@ int apply_prephase(qd_handle *h) {
     hipLaunchKernelGGL(k_f32_to_f64, grid1(h->npt), dim3(256), 0, h->stream, h->psi32, p64, h->npt);
   }
   if (h->variant == QD_VARIANT_CUDA)
-    hipLaunchKernelGGL(k_prephase_cuda, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dt, h->npt);
+    hipLaunchKernelGGL(k_prephase_cuda, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dcol_vidx, h->n12,
+                       h->dt, h->npt);
   else
-    hipLaunchKernelGGL(k_prephase_cpu, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dt, h->npt);
+    hipLaunchKernelGGL(k_prephase_cpu, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dcol_vidx, h->n12,
+                       h->dt, h->npt);
   QD_HIP(hipGetLastError());
   if (tmp) {
     hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, p64, h->psi32, h->npt);
@@ -342,80 +398,349 @@ int step_impl(qd_handle *h) {
   if (h->pipe == QD_PIPE_FUSED) {
     // CUDA propagator: dump at the start of the step (evolutionCUDA2.cu:503),
     // fused with the potential phase; CPU propagator: dump after the step.
-    QD_RC(tick(h, TK_THETA, true));
+    QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, true, cuda && dump ? 2 : 1, 0));
-    QD_RC(tock(h, TK_THETA));
-    QD_RC(tick(h, TK_FFTR1, true));
-    QD_RC(launch_fft<T>(h, 0, FFT_PRE_ROT | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
-    QD_RC(tock(h, TK_FFTR1));
-    QD_RC(tick(h, TK_FFTR2, true));
-    QD_RC(launch_fft<T>(h, 1, FFT_FWD | FFT_KIN | FFT_INV | FFT_POST_ROT, lpsi));
-    QD_RC(tock(h, TK_FFTR2));
-    QD_RC(tick(h, TK_THETA, true));
+    QD_RC(mark(h, TK_FFTR1));
+    QD_RC(launch_rpass<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
+    QD_RC(mark(h, TK_FFTR2));
+    QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
+    QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, false, 0, !cuda && dump ? 1 : 0));
-    QD_RC(tock(h, TK_THETA));
   } else {
     // one kernel per reference operator (timeEvol.C:619-647, dump_psi :713-735)
-    QD_RC(tick(h, TK_OTHER, true));
+    QD_RC(mark(h, TK_OTHER));
     if (cuda && dump)
       hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
     hipLaunchKernelGGL(k_vphase<T>, grid1(h->npt), dim3(256), 0, h->stream, psi,
-                       h->prec == QD_PREC_FP64 ? h->V : nullptr, (const C *)h->expV32, h->dt, h->npt);
-    QD_RC(tock(h, TK_OTHER));
-    QD_RC(tick(h, TK_THETA, true));
+                       h->prec == QD_PREC_FP64 ? h->V : nullptr, (const C *)h->expV32, h->dcol_vidx, h->n12, h->dt,
+                       h->npt);
+    QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, true, 0, 0));
-    QD_RC(tock(h, TK_THETA));
-    QD_RC(tick(h, TK_OTHER, true));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->rotR, (const C *)h->rotr,
-                       h->n1, h->n2, h->nlp);
-    QD_RC(tock(h, TK_OTHER));
-    QD_RC(tick(h, TK_FFTR1, true));
-    QD_RC(launch_fft<T>(h, 0, FFT_FWD, lpsi));
-    QD_RC(tock(h, TK_FFTR1));
-    QD_RC(tick(h, TK_FFTR2, true));
-    QD_RC(launch_fft<T>(h, 1, FFT_FWD, lpsi));
-    QD_RC(tock(h, TK_FFTR2));
-    QD_RC(tick(h, TK_OTHER, true));
+    QD_RC(mark(h, TK_OTHER));
+    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->dR, (const C *)h->dr,
+                       h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
+    QD_RC(mark(h, TK_FFTR1));
+    QD_RC(launch_rpass<T>(h, FFT_FWD, lpsi));
+    QD_RC(mark(h, TK_FFTR2));
+    QD_RC(launch_rline<T>(h, FFT_FWD, lpsi));
+    QD_RC(mark(h, TK_OTHER));
     hipLaunchKernelGGL(k_kin2d<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->kinR,
                        (const C *)h->kinr, h->n1, h->n2, h->nlp);
-    QD_RC(tock(h, TK_OTHER));
-    QD_RC(tick(h, TK_FFTR2, true));
-    QD_RC(launch_fft<T>(h, 1, FFT_INV, lpsi));
-    QD_RC(tock(h, TK_FFTR2));
-    QD_RC(tick(h, TK_FFTR1, true));
-    QD_RC(launch_fft<T>(h, 0, FFT_INV, lpsi));
-    QD_RC(tock(h, TK_FFTR1));
-    QD_RC(tick(h, TK_OTHER, true));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->rotR, (const C *)h->rotr,
-                       h->n1, h->n2, h->nlp);
-    QD_RC(tock(h, TK_OTHER));
-    QD_RC(tick(h, TK_THETA, true));
+    QD_RC(mark(h, TK_FFTR2));
+    QD_RC(launch_rline<T>(h, FFT_INV, lpsi));
+    QD_RC(mark(h, TK_FFTR1));
+    QD_RC(launch_rpass<T>(h, FFT_INV, lpsi));
+    QD_RC(mark(h, TK_OTHER));
+    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->dR, (const C *)h->dr,
+                       h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
+    QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, false, 0, 0));
-    QD_RC(tock(h, TK_THETA));
     if (!cuda && dump) {
-      QD_RC(tick(h, TK_OTHER, true));
+      QD_RC(mark(h, TK_OTHER));
       hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
-      QD_RC(tock(h, TK_OTHER));
     }
   }
   QD_HIP(hipGetLastError());
   if (h->do_crp) {
-    QD_RC(tick(h, TK_FLUX, true));
+    QD_RC(mark(h, TK_FLUX));
     hipLaunchKernelGGL(k_flux_extract<T>, grid1(h->nx), dim3(256), 0, h->stream, (const C *)psi, h->bpsi, h->bdpsi,
-                       h->n1, h->n2, h->nth, h->surf, h->dr2, h->stencil, h->slot);
+                       h->n1, h->n2, h->ndcol, h->surf, h->dr2, h->stencil, h->slot);
     QD_HIP(hipGetLastError());
-    QD_RC(tock(h, TK_FLUX));
     h->slot++;
     if (h->slot == h->tblock) QD_RC(flush_flux(h));
   }
   return QD_OK;
 }
 
-int build_tables(qd_handle *h, const qd_problem *p) {
-  const int n1 = h->n1, n2 = h->n2, M = h->M, nth = h->nth;
-  const bool cuda = h->variant == QD_VARIANT_CUDA;
-  const double dt = h->dt;
+// ---- construction ----
+int validate_base(const qd_problem *p, bool need_theta) {
+  if (!p) return fail(QD_ERR_NULL, "problem is NULL");
+  if (!p->r1.r || !p->r2.r) return fail(QD_ERR_NULL, "r1.r or r2.r is NULL");
+  if (need_theta && (!p->pot || !p->psi || !p->theta.x || !p->theta.w || !p->theta.legendre))
+    return fail(QD_ERR_NULL, "a required array (pot, psi, theta.x/w/legendre) is NULL");
+  if (p->r1.n <= 0 || p->r2.n <= 0) return fail(QD_ERR_ARG, "grid sizes must be positive");
+  if (need_theta && (p->theta.n <= 0 || p->theta.m < 0)) return fail(QD_ERR_ARG, "theta sizes must be positive");
+  if (p->r1.n % 2 || p->r2.n % 2) return fail(QD_ERR_ODD_N, "r1.n and r2.n must be even (FFT momentum grid)");
+  if (p->time.total_steps < 0) return fail(QD_ERR_ARG, "total_steps < 0");
+  if (!(p->r1.mass > 0) || !(p->r2.mass > 0) || !(p->r1.dr > 0) || !(p->r2.dr > 0))
+    return fail(QD_ERR_ARG, "masses and spacings must be positive");
+  return QD_OK;
+}
+
+int set_options(qd_handle *h, const qd_options *o) {
+  qd_options opt{};
+  if (o) opt = *o;
+  if (opt.precision != QD_PREC_FP64 && opt.precision != QD_PREC_MIXED) return fail(QD_ERR_ARG, "bad precision");
+  if (opt.pipeline != QD_PIPE_FUSED && opt.pipeline != QD_PIPE_PLAIN) return fail(QD_ERR_ARG, "bad pipeline");
+  if (opt.variant != QD_VARIANT_CPU && opt.variant != QD_VARIANT_CUDA) return fail(QD_ERR_ARG, "bad variant");
+  if (opt.theta_split != 0 && opt.theta_split != 2 && opt.theta_split != 3)
+    return fail(QD_ERR_ARG, "theta_split must be 0, 2 or 3");
+  if (opt.flux_block < 0) return fail(QD_ERR_ARG, "flux_block < 0");
+  h->dev = opt.device;
+  h->prec = opt.precision;
+  h->pipe = opt.pipeline;
+  h->variant = opt.variant;
+  h->tblock = opt.flux_block > 0 ? opt.flux_block : 64;
+  h->crp_every = opt.crp_every;
+  h->profile = opt.profile;
+  h->nsplit = opt.theta_split == 2 ? 2 : 3;
+  return QD_OK;
+}
+
+// host-side checks of the flux parameters; fills the stencil
+int setup_flux(qd_handle *h, const qd_problem *p) {
+  h->do_crp = p->crp.calculate_CRP != 0;
+  h->ne = h->do_crp ? p->crp.n_energies : 0;
+  if (!h->do_crp) return QD_OK;
+  if (!p->crp.energies || !p->crp.eta_sq) return fail(QD_ERR_NULL, "CRP arrays are NULL");
+  if (h->ne <= 0) return fail(QD_ERR_ARG, "n_energies must be positive");
+  const int np = p->crp.n_gradient_points;
+  if (np < 3 || np > 61 || np % 2 == 0) return fail(QD_ERR_STENCIL, "NPoints error");
+  StencilArg &st = h->stencil;
+  st.h = np / 2;
+  static const double t3[] = {1}, t5[] = {8, -1}, t7[] = {45, -9, 1}, t9[] = {672, -168, 32, -3},
+                      t11[] = {2100, -600, 150, -25, 2};
+  const double *tab = np == 3 ? t3 : np == 5 ? t5 : np == 7 ? t7 : np == 9 ? t9 : np == 11 ? t11 : nullptr;
+  const double den = np == 3 ? 2 : np == 5 ? 12 : np == 7 ? 60 : np == 9 ? 840 : np == 11 ? 2520 : 1;
+  if (tab) {
+    for (int j = 0; j < st.h; j++) st.num[j] = tab[j];
+    st.den = den;
+  } else {  // c_j = (-1)^(j+1) (h!)^2 / (j (h-j)! (h+j)!)  (gradients2.cu:18-35)
+    for (int j = 1; j <= st.h; j++) {
+      long double c = 1.0L;
+      for (int i = 1; i <= j; i++) c *= (long double)(st.h - j + i) / (long double)(st.h + i);
+      c /= j;
+      st.num[j - 1] = (double)((j % 2) ? c : -c);
+    }
+    st.den = 1;
+  }
+  h->surf = p->crp.n_dividing_surface - (h->variant == QD_VARIANT_CUDA ? 0 : 1);
+  if (h->surf - st.h < 0 || h->surf + st.h >= h->n2) return fail(QD_ERR_SURFACE, "surface stencil outside r grid");
+  return QD_OK;
+}
+
+// Upload a complex table given in FP64 as the handle's precision.
+int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
+  if (h->prec == QD_PREC_FP64) {
+    double *d = nullptr;
+    QD_RC(upload(&d, v.data(), v.size()));
+    *dst = d;
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    float *d = nullptr;
+    QD_RC(upload(&d, f.data(), f.size()));
+    *dst = d;
+  }
+  return QD_OK;
+}
+
+int create_batch(qd_handle *h, const BatchSpec &S) {
+  const qd_problem *p = S.base;
+  const int n1 = p->r1.n, n2 = p->r2.n;
+  h->n1 = n1;
+  h->n2 = n2;
+  h->n12 = (size_t)n1 * n2;
+  h->dt = p->time.time_step;
+  h->dr1 = p->r1.dr;
+  h->dr2 = p->r2.dr;
+  h->mu2 = p->r2.mass;
+  h->steps = p->time.steps;
+  h->pending_prephase = p->time.steps == 0;
+  h->has_dump = p->dump1 && p->dump2;
+  h->has_mix = !S.umat.empty();
+  QD_RC(setup_flux(h, p));
+  if (h->pipe == QD_PIPE_PLAIN && (S.blocks.size() != 1 || h->has_mix))
+    return fail(QD_ERR_ARG, "the plain pipeline propagates a single J = 0 block");
+
+  // blocks and column tables
+  h->nb = (int)S.blocks.size();
+  int dcol = 0, fcol = 0, maxn = 0;
+  std::vector<long long> dcol_vidx;
+  std::vector<double> dcol_w;
+  std::vector<long long> voff(S.vsets.size(), -1);
+  long long vtot = 0;
+  for (const HostBlock &b : S.blocks) {
+    const MatSet &M = S.mats[b.mat];
+    BlockDesc d;
+    d.J = b.J;
+    d.K = b.K;
+    d.nd = M.nd;
+    d.nf = M.nf;
+    d.dcol0 = dcol;
+    d.fcol0 = fcol;
+    d.mat = b.mat;
+    d.pad = 0;
+    h->hblocks.push_back(d);
+    if (voff[b.vset] < 0) {
+      voff[b.vset] = vtot;
+      vtot += (long long)h->n12 * M.nd;
+    }
+    for (int k = 0; k < M.nd; k++) {
+      dcol_vidx.push_back(voff[b.vset] + (long long)h->n12 * k);
+      dcol_w.push_back(M.w[k]);
+    }
+    dcol += M.nd;
+    fcol += M.nf;
+    maxn = std::max(maxn, std::max(M.nd, M.nf));
+  }
+  h->ndcol = dcol;
+  h->nfcol = fcol;
+  h->npt = h->n12 * dcol;
+  h->nlp = h->n12 * fcol;
+  h->nx = (size_t)n1 * dcol;
+  if ((int)S.fcol_j.size() != fcol || (int)S.fcol_lamval.size() != fcol)
+    return fail(QD_ERR_ARG, "internal: column tables do not match the blocks");
+
+  // FFT plans and launch shapes
+  if (!factor_plan(n1, h->planR)) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n1) + " has a prime factor > 5");
+  if (!factor_plan(n2, h->planr)) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n2) + " has a prime factor > 5");
+  const size_t elem = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+  int gmax = 1;
+  for (const Group &g : S.groups) gmax = std::max(gmax, (int)g.cols.size());
+  if (gmax > RP_GMAX) return fail(QD_ERR_ARG, "mixing groups larger than 32 (J > 31) are not supported");
+  std::vector<int> bu0, ucols;
+  std::vector<RUnit> units;
+  {
+    const int mb = max_butterflies(h->planR);
+    const int tpl = std::max(1, pow2ceil((mb + FFT_BPT - 1) / FFT_BPT));
+    if (tpl > RP_THREADS) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n1) + " too large");
+    int ml = std::min(RP_THREADS / tpl, 64);
+    const int ml_lds = (int)(160 * 1024 / ((size_t)n1 * elem));
+    const int ml_64k = (int)(65536 / ((size_t)n1 * elem));
+    if (std::min(ml, ml_lds) < gmax) return fail(QD_ERR_ARG, "R grid too large for the K-mixing groups");
+    ml = std::min(ml, ml_64k >= gmax ? ml_64k : ml_lds);
+    h->rp_tpl = tpl;
+    int lines = 0, nu = 0;
+    bu0.push_back(0);
+    for (const Group &g : S.groups) {
+      const int gs = (int)g.cols.size();
+      const int c0 = (int)ucols.size();
+      for (int c : g.cols) ucols.push_back(c);
+      for (int r = 0; r < n2; r++) {
+        if (lines + gs > ml || nu == 64) {
+          bu0.push_back((int)units.size());
+          lines = 0;
+          nu = 0;
+        }
+        RUnit u;
+        u.col0 = c0;
+        u.g = gs;
+        u.u0 = g.u0;
+        u.r = r;
+        units.push_back(u);
+        lines += gs;
+        nu++;
+      }
+    }
+    bu0.push_back((int)units.size());
+    h->nbundle = (int)bu0.size() - 1;
+    h->rp_lds = (size_t)ml * n1 * elem;
+  }
+  {
+    const int mb = max_butterflies(h->planr);
+    int lpb = 16;
+    while (lpb > 1 && ((256 / lpb) * FFT_BPT < mb || (size_t)lpb * n2 * elem > 65536)) lpb /= 2;
+    if ((256 / lpb) * FFT_BPT < mb || (size_t)lpb * n2 * elem > 65536)
+      return fail(QD_ERR_ARG, "FFT length " + std::to_string(n2) + " too large");
+    h->rl_lpb = lpb;
+    h->rl_blocks = (int)(((size_t)n1 * fcol + lpb - 1) / lpb);
+    h->rl_lds = (size_t)lpb * n2 * elem;
+  }
+  // theta matrices: padded shapes shared by all sets
+  h->ncb = (h->prec == QD_PREC_MIXED && maxn <= 64) ? 4 : 8;
+  const int colw = h->prec == QD_PREC_FP64 ? TG_COLS : h->ncb * 16;
+  h->mat_Npad = (maxn + colw - 1) / colw * colw;
+  h->col_tiles = h->mat_Npad / colw;
+  const int kc = h->prec == QD_PREC_FP64 ? TG_KC64 : TG_KC16;
+  h->mat_Kpad = (maxn + kc - 1) / kc * kc;
+
+  // ---- everything above is host-side validation; device work starts here ----
+  QD_HIP(hipSetDevice(h->dev));
+  QD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto &e : h->ev) QD_HIP(hipEventCreate(&e));
+  if (h->profile) {
+    h->marks.assign(512, nullptr);
+    h->mark_cls.assign(512, -1);
+    for (auto &e : h->marks) QD_HIP(hipEventCreate(&e));
+  }
+  if (h->rp_lds > 65536) {
+    if (h->prec == QD_PREC_FP64)
+      QD_HIP(hipFuncSetAttribute((const void *)k_rpass<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)h->rp_lds));
+    else
+      QD_HIP(hipFuncSetAttribute((const void *)k_rpass<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)h->rp_lds));
+  }
+  QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
+  QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));
+  QD_RC(upload(&h->dcol_w, dcol_w.data(), dcol_w.size()));
+  QD_RC(upload(&h->bundle_u0, bu0.data(), bu0.size()));
+  QD_RC(upload(&h->units, units.data(), units.size()));
+  QD_RC(upload(&h->unit_cols, ucols.data(), ucols.size()));
+  if (h->prec == QD_PREC_FP64) {
+    double *u = nullptr;
+    QD_RC(upload(&u, S.umat.data(), S.umat.size()));
+    h->umat = u;
+  } else {
+    std::vector<float> uf(S.umat.begin(), S.umat.end());
+    float *u = nullptr;
+    QD_RC(upload(&u, uf.data(), uf.size()));
+    h->umat = u;
+  }
+
+  // psi: blocks back to back
+  {
+    std::vector<double> host(2 * h->npt);
+    for (int b = 0; b < h->nb; b++) {
+      const BlockDesc &d = h->hblocks[b];
+      memcpy(host.data() + 2 * h->n12 * d.dcol0, S.blocks[b].psi, sizeof(double) * 2 * h->n12 * d.nd);
+    }
+    if (h->prec == QD_PREC_FP64) {
+      QD_RC(upload(&h->psi64, reinterpret_cast<const double2 *>(host.data()), h->npt));
+      QD_HIP(hipMalloc((void **)&h->lpsi64, sizeof(double2) * h->nlp));
+    } else {
+      double2 *tmp = nullptr;
+      QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(host.data()), h->npt));
+      QD_HIP(hipMalloc((void **)&h->psi32, sizeof(float2) * h->npt));
+      QD_HIP(hipMalloc((void **)&h->lpsi32, sizeof(float2) * h->nlp));
+      hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, tmp, h->psi32, h->npt);
+      QD_HIP(hipStreamSynchronize(h->stream));
+      QD_HIP(hipFree(tmp));
+    }
+  }
+  // potential sets (FP64; the mixed path adds an FP32 exp(-i V dt) table)
+  {
+    std::vector<double> vh((size_t)vtot);
+    for (size_t s = 0; s < S.vsets.size(); s++) {
+      if (voff[s] < 0) continue;
+      int nd = 0;
+      for (const HostBlock &b : S.blocks)
+        if (b.vset == (int)s) nd = S.mats[b.mat].nd;
+      memcpy(vh.data() + voff[s], S.vsets[s], sizeof(double) * h->n12 * nd);
+    }
+    QD_RC(upload(&h->V, vh.data(), vh.size()));
+    if (h->prec == QD_PREC_MIXED) {
+      std::vector<float> e(2 * vh.size());
+      for (size_t x = 0; x < vh.size(); x++) {
+        const double a = -h->dt * vh[x];
+        e[2 * x] = (float)cos(a);
+        e[2 * x + 1] = (float)sin(a);
+      }
+      float *d = nullptr;
+      QD_RC(upload(&d, e.data(), e.size()));
+      h->expV32 = reinterpret_cast<float2 *>(d);
+    }
+  }
+  if (h->has_dump) {
+    QD_RC(upload(&h->d1, p->dump1, n1));
+    QD_RC(upload(&h->d2, p->dump2, n2));
+    std::vector<float> f1(p->dump1, p->dump1 + n1), f2(p->dump2, p->dump2 + n2);
+    QD_RC(upload(&h->d1f, f1.data(), f1.size()));
+    QD_RC(upload(&h->d2f, f2.data(), f2.size()));
+  }
+
   // radial tables (MatlabStructures.C:17-36, fftwinterface.C:110-125)
+  const bool cuda = h->variant == QD_VARIANT_CUDA;
   auto momentum = [](int nm, double xl, std::vector<double> &pm) {
     pm.resize(nm);
     const double two_Pi_xl = 2.0 * M_PI / xl;
@@ -436,28 +761,41 @@ int build_tables(qd_handle *h, const qd_problem *p) {
   momentum(n2, n2 * p->r2.dr, p2v);
   std::vector<double> I1(n1), I2(n2);
   for (int i = 0; i < n1; i++) {
-    const double r = cuda ? p->r1.r[0] + i * p->r1.dr : p->r1.r[i];
+    const double r = cuda ? p->r1.r[0] + i * p->r1.dr : p->r1.r[i];  // cumath.h:153-160
     I1[i] = 1.0 / ((p->r1.mass + p->r1.mass) * r * r);
   }
   for (int j = 0; j < n2; j++) {
     const double r = cuda ? p->r2.r[0] + j * p->r2.dr : p->r2.r[j];
     I2[j] = 1.0 / ((p->r2.mass + p->r2.mass) * r * r);
   }
-  // complex tables in FP64
-  std::vector<double> rotR(2 * (size_t)M * n1), rotr(2 * (size_t)M * n2), kinR(2 * n1), kinr(2 * n2), twR(2 * n1),
-      twr(2 * n2);
-  for (int l = 0; l < M; l++) {
-    for (int i = 0; i < n1; i++) {
-      const double a = (-dt / 2) * (l * (l + 1) * I1[i]);
-      rotR[2 * ((size_t)l * n1 + i)] = cos(a);
-      rotR[2 * ((size_t)l * n1 + i) + 1] = sin(a);
+  const double dt = h->dt;
+  // D_R/2 per distinct lambda, D_r/2 per j
+  std::map<long long, int> lam_index;
+  std::vector<long long> lams;
+  std::vector<int> fcol_lam(fcol);
+  for (int c = 0; c < fcol; c++) {
+    auto it = lam_index.find(S.fcol_lamval[c]);
+    if (it == lam_index.end()) {
+      it = lam_index.emplace(S.fcol_lamval[c], (int)lams.size()).first;
+      lams.push_back(S.fcol_lamval[c]);
     }
-    for (int j = 0; j < n2; j++) {
-      const double a = (-dt / 2) * (l * (l + 1) * I2[j]);
-      rotr[2 * ((size_t)l * n2 + j)] = cos(a);
-      rotr[2 * ((size_t)l * n2 + j) + 1] = sin(a);
-    }
+    fcol_lam[c] = it->second;
   }
+  const int njr = S.jmax + 1;
+  std::vector<double> dRt(2 * lams.size() * n1), drt(2 * (size_t)njr * n2), kinR(2 * n1), kinr(2 * n2), twR(2 * n1),
+      twr(2 * n2);
+  for (size_t q = 0; q < lams.size(); q++)
+    for (int i = 0; i < n1; i++) {
+      const double a = (-dt / 2) * ((double)lams[q] * I1[i]);
+      dRt[2 * (q * n1 + i)] = cos(a);
+      dRt[2 * (q * n1 + i) + 1] = sin(a);
+    }
+  for (int j = 0; j < njr; j++)
+    for (int r = 0; r < n2; r++) {
+      const double a = (-dt / 2) * ((double)j * (j + 1) * I2[r]);
+      drt[2 * ((size_t)j * n2 + r)] = cos(a);
+      drt[2 * ((size_t)j * n2 + r) + 1] = sin(a);
+    }
   for (int i = 0; i < n1; i++) {
     const double T = p1v[i] * p1v[i] / (2 * p->r1.mass);
     kinR[2 * i] = cos(-dt * T) / n1;
@@ -474,215 +812,67 @@ int build_tables(qd_handle *h, const qd_problem *p) {
     twr[2 * j] = (double)cosl(ang);
     twr[2 * j + 1] = (double)sinl(ang);
   }
-  auto put = [&](void **dst, const std::vector<double> &v) -> int {
-    if (h->prec == QD_PREC_FP64) {
-      double *d = nullptr;
-      QD_RC(upload(&d, v.data(), v.size()));
-      *dst = d;
-    } else {
-      std::vector<float> f(v.begin(), v.end());
-      float *d = nullptr;
-      QD_RC(upload(&d, f.data(), f.size()));
-      *dst = d;
-    }
-    return QD_OK;
-  };
-  QD_RC(put(&h->rotR, rotR));
-  QD_RC(put(&h->rotr, rotr));
-  QD_RC(put(&h->kinR, kinR));
-  QD_RC(put(&h->kinr, kinr));
-  QD_RC(put(&h->twR, twR));
-  QD_RC(put(&h->twr, twr));
+  QD_RC(put_table(h, &h->dR, dRt));
+  QD_RC(put_table(h, &h->dr, drt));
+  QD_RC(put_table(h, &h->kinR, kinR));
+  QD_RC(put_table(h, &h->kinr, kinr));
+  QD_RC(put_table(h, &h->twR, twR));
+  QD_RC(put_table(h, &h->twr, twr));
+  QD_RC(upload(&h->fcol_lam, fcol_lam.data(), fcol_lam.size()));
+  QD_RC(upload(&h->fcol_j, S.fcol_j.data(), S.fcol_j.size()));
 
-  // theta matrices (setup_weight_legendre, timeEvol.C:737-760; legendre, MatlabStructures.C:49)
-  const double *leg = p->theta.legendre;
-  const double *w = p->theta.w;
-  std::vector<double> Wf((size_t)nth * M), Wb((size_t)M * nth);  // Wf[k][l], Wb[l][k]
-  for (int l = 0; l < M; l++) {
-    const double f = l + 0.5;
-    for (int k = 0; k < nth; k++) {
-      Wf[(size_t)k * M + l] = f * w[k] * leg[l + (size_t)M * k];
-      Wb[(size_t)l * nth + k] = leg[l + (size_t)M * k];
+  // theta matrices, packed per set
+  const int nmat = (int)S.mats.size();
+  const int Kpad = h->mat_Kpad, Npad = h->mat_Npad;
+  if (h->prec == QD_PREC_FP64) {
+    h->mat_stride = (long long)Kpad * Npad;
+    std::vector<double> F((size_t)nmat * h->mat_stride, 0.0), B((size_t)nmat * h->mat_stride, 0.0);
+    for (int s = 0; s < nmat; s++) {
+      const MatSet &M = S.mats[s];
+      double *f = F.data() + (size_t)s * h->mat_stride, *bq = B.data() + (size_t)s * h->mat_stride;
+      for (int k = 0; k < M.nd; k++)
+        for (int j = 0; j < M.nf; j++) f[(size_t)k * Npad + j] = M.Wf[(size_t)k * M.nf + j];
+      for (int j = 0; j < M.nf; j++)
+        for (int k = 0; k < M.nd; k++) bq[(size_t)j * Npad + k] = M.Pb[(size_t)j * M.nd + k];
     }
-  }
-  auto pack64 = [&](const std::vector<double> &B, int K, int N, double **dst, int &Npad) -> int {
-    Npad = (N + TG_COLS - 1) / TG_COLS * TG_COLS;
-    const int Kpad = (K + TG_KC64 - 1) / TG_KC64 * TG_KC64;
-    std::vector<double> P((size_t)Kpad * Npad, 0.0);
-    for (int k = 0; k < K; k++)
-      for (int n = 0; n < N; n++) P[(size_t)k * Npad + n] = B[(size_t)k * N + n];
-    return upload(dst, P.data(), P.size());
-  };
-  auto pack16 = [&](const std::vector<double> &B, int K, int N, __bf16 **dst, int &Npad, int &Kpad) -> int {
-    Npad = (N + TG_COLS - 1) / TG_COLS * TG_COLS;
-    Kpad = (K + TG_KC16 - 1) / TG_KC16 * TG_KC16;
-    const int NS = 3;  // pieces stored; a 2-piece kernel reads the first two
-    std::vector<uint16_t> P((size_t)NS * Npad * Kpad, 0);
-    for (int k = 0; k < K; k++)
-      for (int n = 0; n < N; n++) {
-        double r = B[(size_t)k * N + n];
-        for (int s = 0; s < NS; s++) {
-          const uint16_t b = bf16_bits((float)r);
-          P[((size_t)s * Npad + n) * Kpad + k] = b;
-          r -= bf16_value(b);
+    double *df = nullptr, *db = nullptr;
+    QD_RC(upload(&df, F.data(), F.size()));
+    QD_RC(upload(&db, B.data(), B.size()));
+    h->Wf = df;
+    h->Wb = db;
+  } else {
+    const int NS = 3;  // pieces stored; the two-piece kernel reads the first two
+    h->mat_stride = (long long)NS * Npad * Kpad;
+    std::vector<uint16_t> F((size_t)nmat * h->mat_stride, 0), B((size_t)nmat * h->mat_stride, 0);
+    auto split = [&](std::vector<uint16_t> &dst, int s, int K, int N, const std::vector<double> &src, bool transposed) {
+      for (int k = 0; k < K; k++)
+        for (int n = 0; n < N; n++) {
+          double r = transposed ? src[(size_t)k * N + n] : src[(size_t)k * N + n];
+          for (int q = 0; q < NS; q++) {
+            const uint16_t b = bf16_bits((float)r);
+            dst[(size_t)s * h->mat_stride + ((size_t)q * Npad + n) * Kpad + k] = b;
+            r -= bf16_value(b);
+          }
         }
-      }
-    QD_HIP(hipMalloc((void **)dst, P.size() * 2));
-    QD_HIP(hipMemcpy(*dst, P.data(), P.size() * 2, hipMemcpyHostToDevice));
-    return QD_OK;
-  };
-  if (h->prec == QD_PREC_FP64) {
-    QD_RC(pack64(Wf, nth, M, &h->Wf64, h->Wf_Npad));
-    QD_RC(pack64(Wb, M, nth, &h->Wb64, h->Wb_Npad));
-  } else {
-    QD_RC(pack16(Wf, nth, M, &h->Wf16, h->Wf_Npad, h->Wf_Kpad));
-    QD_RC(pack16(Wb, M, nth, &h->Wb16, h->Wb_Npad, h->Wb_Kpad));
-  }
-  return QD_OK;
-}
-
-int validate(const qd_problem *p) {
-  if (!p) return fail(QD_ERR_NULL, "problem is NULL");
-  if (!p->pot || !p->psi || !p->r1.r || !p->r2.r || !p->theta.x || !p->theta.w || !p->theta.legendre)
-    return fail(QD_ERR_NULL, "a required array (pot, psi, r1.r, r2.r, theta.x/w/legendre) is NULL");
-  if (p->r1.n <= 0 || p->r2.n <= 0 || p->theta.n <= 0 || p->theta.m < 0)
-    return fail(QD_ERR_ARG, "grid sizes must be positive");
-  if (p->r1.n % 2 || p->r2.n % 2) return fail(QD_ERR_ODD_N, "r1.n and r2.n must be even (FFT momentum grid)");
-  if (p->time.total_steps < 0) return fail(QD_ERR_ARG, "total_steps < 0");
-  if (!(p->r1.mass > 0) || !(p->r2.mass > 0) || !(p->r1.dr > 0) || !(p->r2.dr > 0))
-    return fail(QD_ERR_ARG, "masses and spacings must be positive");
-  return QD_OK;
-}
-
-void release(qd_handle *h) {
-  if (!h) return;
-  void *bufs[] = {h->psi64, h->lpsi64, h->psi32, h->lpsi32, h->V, h->expV32, h->d1, h->d2, h->d1f, h->d2f,
-                  h->rotR, h->rotr, h->kinR, h->kinr, h->twR, h->twr, h->Wf64, h->Wb64, h->Wf16, h->Wb16,
-                  h->bpsi, h->bdpsi, h->fai, h->dfai, h->energies, h->eta_sq, h->w, h->crp, h->partial};
-  for (void *b : bufs)
-    if (b) (void)hipFree(b);
-  for (auto &e : h->ev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto &e : h->marks)
-    if (e) (void)hipEventDestroy(e);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
-}
-
-int create_impl(const qd_problem *p, const qd_options *o, qd_handle *h) {
-  QD_RC(validate(p));
-  qd_options opt{};
-  if (o) opt = *o;
-  if (opt.precision != QD_PREC_FP64 && opt.precision != QD_PREC_MIXED) return fail(QD_ERR_ARG, "bad precision");
-  if (opt.pipeline != QD_PIPE_FUSED && opt.pipeline != QD_PIPE_PLAIN) return fail(QD_ERR_ARG, "bad pipeline");
-  if (opt.variant != QD_VARIANT_CPU && opt.variant != QD_VARIANT_CUDA) return fail(QD_ERR_ARG, "bad variant");
-  h->dev = opt.device;
-  h->prec = opt.precision;
-  h->pipe = opt.pipeline;
-  h->variant = opt.variant;
-  h->tblock = opt.flux_block > 0 ? opt.flux_block : 64;
-  h->crp_every = opt.crp_every;
-  h->profile = opt.profile;
-  h->nsplit = opt.theta_split == 2 ? 2 : 3;
-  if (opt.theta_split != 0 && opt.theta_split != 2 && opt.theta_split != 3)
-    return fail(QD_ERR_ARG, "theta_split must be 0, 2 or 3");
-  h->n1 = p->r1.n;
-  h->n2 = p->r2.n;
-  h->nth = p->theta.n;
-  h->M = p->theta.m + 1;
-  h->n12 = (size_t)h->n1 * h->n2;
-  h->npt = h->n12 * h->nth;
-  h->nlp = h->n12 * h->M;
-  h->nx = (size_t)h->n1 * h->nth;
-  h->dt = p->time.time_step;
-  h->dr1 = p->r1.dr;
-  h->dr2 = p->r2.dr;
-  h->mu2 = p->r2.mass;
-  h->steps = p->time.steps;
-  h->pending_prephase = p->time.steps == 0;
-  h->has_dump = p->dump1 && p->dump2;
-  h->do_crp = p->crp.calculate_CRP != 0;
-  h->ne = h->do_crp ? p->crp.n_energies : 0;
-  if (h->n12 > (size_t)INT32_MAX / 2) return fail(QD_ERR_ARG, "grid too large");
-
-  const size_t elem = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float2);
-  QD_RC(fft_config(h->n1, 0, elem, h->n2 * h->M, h->fR));
-  QD_RC(fft_config(h->n2, 1, elem, h->n1 * h->M, h->fr));
-
-  if (h->do_crp) {
-    if (!p->crp.energies || !p->crp.eta_sq || !p->crp.CRP) return fail(QD_ERR_NULL, "CRP arrays are NULL");
-    if (h->ne <= 0) return fail(QD_ERR_ARG, "n_energies must be positive");
-    const int np = p->crp.n_gradient_points;
-    if (np < 3 || np > 61 || np % 2 == 0) return fail(QD_ERR_STENCIL, "NPoints error");
-    StencilArg &st = h->stencil;
-    st.h = np / 2;
-    static const double t3[] = {1}, t5[] = {8, -1}, t7[] = {45, -9, 1}, t9[] = {672, -168, 32, -3},
-                        t11[] = {2100, -600, 150, -25, 2};
-    const double *tab = np == 3 ? t3 : np == 5 ? t5 : np == 7 ? t7 : np == 9 ? t9 : np == 11 ? t11 : nullptr;
-    const double den = np == 3 ? 2 : np == 5 ? 12 : np == 7 ? 60 : np == 9 ? 840 : np == 11 ? 2520 : 1;
-    if (tab) {
-      for (int j = 0; j < st.h; j++) st.num[j] = tab[j];
-      st.den = den;
-    } else {  // c_j = (-1)^(j+1) (h!)^2 / (j (h-j)! (h+j)!)  (gradients2.cu:18-35)
-      for (int j = 1; j <= st.h; j++) {
-        long double c = 1.0L;
-        for (int i = 1; i <= j; i++) c *= (long double)(st.h - j + i) / (long double)(st.h + i);
-        c /= j;
-        st.num[j - 1] = (double)((j % 2) ? c : -c);
-      }
-      st.den = 1;
+    };
+    for (int s = 0; s < nmat; s++) {
+      const MatSet &M = S.mats[s];
+      split(F, s, M.nd, M.nf, M.Wf, false);  // B of the forward GEMM: [k][j]
+      split(B, s, M.nf, M.nd, M.Pb, false);  // B of the backward GEMM: [j][k]
     }
-    h->surf = p->crp.n_dividing_surface - (h->variant == QD_VARIANT_CUDA ? 0 : 1);
-    if (h->surf - st.h < 0 || h->surf + st.h >= h->n2) return fail(QD_ERR_SURFACE, "surface stencil outside r grid");
+    void *df = nullptr, *db = nullptr;
+    QD_HIP(hipMalloc(&df, F.size() * 2));
+    QD_HIP(hipMemcpy(df, F.data(), F.size() * 2, hipMemcpyHostToDevice));
+    QD_HIP(hipMalloc(&db, B.size() * 2));
+    QD_HIP(hipMemcpy(db, B.data(), B.size() * 2, hipMemcpyHostToDevice));
+    h->Wf = df;
+    h->Wb = db;
   }
 
-  // ---- everything above is host-side validation; device work starts here ----
-  QD_HIP(hipSetDevice(h->dev));
-  QD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  for (auto &e : h->ev) QD_HIP(hipEventCreate(&e));
-  if (h->profile) {
-    h->marks.assign(512, nullptr);
-    h->mark_cls.assign(512, -1);
-    for (auto &e : h->marks) QD_HIP(hipEventCreate(&e));
-  }
-
-  const double *psi_in = p->psi;
-  if (h->prec == QD_PREC_FP64) {
-    QD_RC(upload(&h->psi64, reinterpret_cast<const double2 *>(psi_in), h->npt));
-    QD_HIP(hipMalloc((void **)&h->lpsi64, sizeof(double2) * h->nlp));
-  } else {
-    double2 *tmp = nullptr;
-    QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(psi_in), h->npt));
-    QD_HIP(hipMalloc((void **)&h->psi32, sizeof(float2) * h->npt));
-    QD_HIP(hipMalloc((void **)&h->lpsi32, sizeof(float2) * h->nlp));
-    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, tmp, h->psi32, h->npt);
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipFree(tmp));
-  }
-  QD_RC(upload(&h->V, p->pot, h->npt));
-  if (h->prec == QD_PREC_MIXED) {
-    std::vector<float> e(2 * h->npt);
-    for (size_t x = 0; x < h->npt; x++) {
-      const double a = -h->dt * p->pot[x];
-      e[2 * x] = (float)cos(a);
-      e[2 * x + 1] = (float)sin(a);
-    }
-    float *d = nullptr;
-    QD_RC(upload(&d, e.data(), e.size()));
-    h->expV32 = reinterpret_cast<float2 *>(d);
-  }
-  if (h->has_dump) {
-    QD_RC(upload(&h->d1, p->dump1, h->n1));
-    QD_RC(upload(&h->d2, p->dump2, h->n2));
-    std::vector<float> f1(p->dump1, p->dump1 + h->n1), f2(p->dump2, p->dump2 + h->n2);
-    QD_RC(upload(&h->d1f, f1.data(), f1.size()));
-    QD_RC(upload(&h->d2f, f2.data(), f2.size()));
-  }
-  QD_RC(upload(&h->w, p->theta.w, h->nth));
-  QD_RC(build_tables(h, p));
   if (h->do_crp) {
     QD_RC(upload(&h->energies, p->crp.energies, h->ne));
     QD_RC(upload(&h->eta_sq, p->crp.eta_sq, h->ne));
-    QD_HIP(hipMalloc((void **)&h->crp, sizeof(double) * h->ne));
+    QD_HIP(hipMalloc((void **)&h->crp, sizeof(double) * h->ne * h->nb));
     QD_HIP(hipMalloc((void **)&h->bpsi, sizeof(double2) * h->nx * h->tblock));
     QD_HIP(hipMalloc((void **)&h->bdpsi, sizeof(double2) * h->nx * h->tblock));
     QD_HIP(hipMalloc((void **)&h->fai, sizeof(double2) * h->nx * h->ne));
@@ -693,6 +883,170 @@ int create_impl(const qd_problem *p, const qd_options *o, qd_handle *h) {
   QD_HIP(hipMalloc((void **)&h->partial, sizeof(double) * h->npartial));
   QD_HIP(hipStreamSynchronize(h->stream));
   return QD_OK;
+}
+
+// The reference's J = 0 problem as a batch of one block.
+int spec_from_problem(const qd_problem *p, BatchSpec &S) {
+  QD_RC(validate_base(p, true));
+  const int nth = p->theta.n, M = p->theta.m + 1;
+  S.base = p;
+  S.jmax = M - 1;
+  MatSet ms;
+  ms.nd = nth;
+  ms.nf = M;
+  ms.Wf.resize((size_t)nth * M);
+  ms.Pb.resize((size_t)M * nth);
+  ms.w.assign(p->theta.w, p->theta.w + nth);
+  const double *leg = p->theta.legendre;
+  for (int l = 0; l < M; l++) {
+    const double f = l + 0.5;  // setup_weight_legendre, timeEvol.C:737-760
+    for (int k = 0; k < nth; k++) {
+      ms.Wf[(size_t)k * M + l] = f * p->theta.w[k] * leg[l + (size_t)M * k];
+      ms.Pb[(size_t)l * nth + k] = leg[l + (size_t)M * k];
+    }
+  }
+  S.mats.push_back(ms);
+  S.vsets.push_back(p->pot);
+  HostBlock b;
+  b.J = 0;
+  b.K = 0;
+  b.mat = 0;
+  b.vset = 0;
+  b.psi = p->psi;
+  S.blocks.push_back(b);
+  for (int l = 0; l < M; l++) {
+    S.fcol_lamval.push_back((long long)l * (l + 1));
+    S.fcol_j.push_back(l);
+    Group g;
+    g.cols.push_back(l);
+    S.groups.push_back(g);
+  }
+  return QD_OK;
+}
+
+int spec_from_jk(const qd_jk_problem *q, BatchSpec &S) {
+  if (!q) return fail(QD_ERR_NULL, "problem is NULL");
+  QD_RC(validate_base(&q->base, false));
+  if (q->jmax < 0 || q->n_blocks <= 0 || !q->blocks) return fail(QD_ERR_ARG, "jmax and blocks are required");
+  S.base = &q->base;
+  S.jmax = q->jmax;
+  const int jmax = q->jmax;
+  // theta sets: one per K (the J = 0 block reuses the reference Legendre matrices)
+  std::map<int, int> mat_of_key, vset_of_K;
+  for (int b = 0; b < q->n_blocks; b++) {
+    const qd_jk_block &B = q->blocks[b];
+    if (B.J < 0 || B.K < 0 || B.K > B.J || B.K > jmax) return fail(QD_ERR_ARG, "block needs 0 <= K <= min(J, jmax)");
+    if (!B.pot || !B.psi) return fail(QD_ERR_NULL, "block pot or psi is NULL");
+    const int key = B.J == 0 ? -1 : B.K;
+    if (!mat_of_key.count(key)) {
+      MatSet ms;
+      std::vector<double> x, w, Pb;
+      qdt::assoc_legendre_dvr(B.K, jmax, x, w, Pb);
+      const int n = jmax - B.K + 1;
+      ms.nd = ms.nf = n;
+      ms.w = w;
+      ms.Wf.resize((size_t)n * n);
+      ms.Pb.resize((size_t)n * n);
+      if (key == -1) {  // reference J = 0 matrices on the same Gauss-Legendre grid
+        std::vector<double> L = qdt::legendre_p2(jmax, x);
+        for (int l = 0; l < n; l++)
+          for (int k = 0; k < n; k++) {
+            ms.Wf[(size_t)k * n + l] = (l + 0.5) * w[k] * L[(size_t)l * n + k];
+            ms.Pb[(size_t)l * n + k] = L[(size_t)l * n + k];
+          }
+      } else {
+        for (int j = 0; j < n; j++)
+          for (int k = 0; k < n; k++) {
+            ms.Wf[(size_t)k * n + j] = w[k] * Pb[(size_t)j * n + k];
+            ms.Pb[(size_t)j * n + k] = Pb[(size_t)j * n + k];
+          }
+      }
+      mat_of_key[key] = (int)S.mats.size();
+      S.mats.push_back(ms);
+    }
+    if (!vset_of_K.count(B.K)) {
+      vset_of_K[B.K] = (int)S.vsets.size();
+      S.vsets.push_back(B.pot);
+    }
+    HostBlock hb;
+    hb.J = B.J;
+    hb.K = B.K;
+    hb.mat = mat_of_key[key];
+    hb.vset = vset_of_K[B.K];
+    hb.psi = B.psi;
+    S.blocks.push_back(hb);
+  }
+  // FBR columns: block (J, K), local index a <-> j = K + a
+  std::map<std::pair<int, int>, int> fcol0;  // (J, K) -> first FBR column
+  int fc = 0;
+  for (const HostBlock &hb : S.blocks) {
+    if (fcol0.count({hb.J, hb.K})) return fail(QD_ERR_ARG, "duplicate (J, K) block");
+    fcol0[{hb.J, hb.K}] = fc;
+    fc += S.mats[hb.mat].nf;
+  }
+  S.fcol_lamval.assign(fc, 0);
+  S.fcol_j.assign(fc, 0);
+  std::map<int, std::vector<int>> Ks_of_J;
+  for (const HostBlock &hb : S.blocks) Ks_of_J[hb.J].push_back(hb.K);
+  for (auto &kv : Ks_of_J) {
+    const int J = kv.first;
+    std::vector<int> Ks = kv.second;
+    std::sort(Ks.begin(), Ks.end());
+    for (int j = 0; j <= jmax; j++) {
+      const int g = std::min(J, j) + 1;
+      // the K blocks of this J must be 0..min(J, jmax) for the coupling to close
+      Group grp;
+      for (int K = 0; K < g; K++) {
+        auto it = fcol0.find({J, K});
+        if (it == fcol0.end()) return fail(QD_ERR_ARG, "J > 0 needs every block K = 0..min(J, jmax)");
+        grp.cols.push_back(it->second + (j - K));
+      }
+      if (q->coriolis && g > 1) {
+        std::vector<double> U;
+        std::vector<int> l;
+        qdt::coriolis_eigen(J, j, U, l);
+        grp.u0 = (int)S.umat.size();
+        S.umat.insert(S.umat.end(), U.begin(), U.end());
+        for (int m = 0; m < g; m++) {
+          S.fcol_lamval[grp.cols[m]] = (long long)l[m] * (l[m] + 1);
+          S.fcol_j[grp.cols[m]] = j;
+        }
+      } else {
+        // J = 0, or the centrifugal-sudden limit: (J - j)^2 -> J(J+1) + j(j+1) - 2K^2
+        for (int K = 0; K < g; K++) {
+          S.fcol_lamval[grp.cols[K]] = (long long)J * (J + 1) + (long long)j * (j + 1) - 2LL * K * K;
+          S.fcol_j[grp.cols[K]] = j;
+        }
+        grp.u0 = -1;
+      }
+      if (g > 1 && q->coriolis) {
+        S.groups.push_back(grp);
+      } else {
+        for (int K = 0; K < g; K++) {
+          Group one;
+          one.cols.push_back(grp.cols[K]);
+          S.groups.push_back(one);
+        }
+      }
+    }
+  }
+  return QD_OK;
+}
+
+void release(qd_handle *h) {
+  if (!h) return;
+  void *bufs[] = {h->psi64,    h->lpsi64, h->psi32,     h->lpsi32,   h->V,      h->expV32,   h->dcol_vidx,
+                  h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d1,     h->d2,       h->d1f,
+                  h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
+                  h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
+                  h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : h->marks)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
 int propagate_impl(qd_handle *h, int32_t nsteps) {
@@ -716,10 +1070,14 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
     h->steps++;
     const bool every = h->crp_every > 0 && (h->steps % h->crp_every == 0);
     if (every && (h->cb || h->do_crp)) {
+      std::vector<double> tot;
       const double *c = nullptr;
       if (h->do_crp) {
         QD_RC(compute_crp(h));
-        c = h->host_crp.data();
+        tot.assign(h->ne, 0.0);
+        for (int b = 0; b < h->nb; b++)
+          for (int e = 0; e < h->ne; e++) tot[e] += h->host_crp[(size_t)b * h->ne + e];
+        c = tot.data();
       }
       if (h->cb && h->cb(h->steps, c, h->cb_user)) break;
     }
@@ -756,11 +1114,11 @@ int get_psi_impl(qd_handle *h, double *out) {
 int norm_impl(qd_handle *h, double *out) {
   QD_HIP(hipSetDevice(h->dev));
   if (h->prec == QD_PREC_FP64)
-    hipLaunchKernelGGL(k_norm_partial<double>, dim3(h->npartial), dim3(256), 0, h->stream, h->psi64, h->w, h->n12,
-                       h->npt, h->partial);
+    hipLaunchKernelGGL(k_norm_partial<double>, dim3(h->npartial), dim3(256), 0, h->stream, h->psi64, h->dcol_w,
+                       h->n12, h->npt, h->partial);
   else
-    hipLaunchKernelGGL(k_norm_partial<float>, dim3(h->npartial), dim3(256), 0, h->stream, h->psi32, h->w, h->n12,
-                       h->npt, h->partial);
+    hipLaunchKernelGGL(k_norm_partial<float>, dim3(h->npartial), dim3(256), 0, h->stream, h->psi32, h->dcol_w,
+                       h->n12, h->npt, h->partial);
   QD_HIP(hipGetLastError());
   std::vector<double> part(h->npartial);
   QD_HIP(hipMemcpyAsync(part.data(), h->partial, sizeof(double) * h->npartial, hipMemcpyDeviceToHost, h->stream));
@@ -768,6 +1126,20 @@ int norm_impl(qd_handle *h, double *out) {
   double s = 0;
   for (double v : part) s += v;
   *out = s * h->dr1 * h->dr2;
+  return QD_OK;
+}
+
+int create_from_spec(const BatchSpec &S, const qd_options *options, qd_handle **out) {
+  qd_handle *h = new (std::nothrow) qd_handle;
+  if (!h) return fail(QD_ERR_NOMEM, "out of host memory");
+  int rc = set_options(h, options);
+  if (rc == QD_OK) rc = create_batch(h, S);
+  if (rc != QD_OK) {
+    release(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
   return QD_OK;
 }
 
@@ -780,15 +1152,26 @@ const char *qd_last_error(void) { return g_last_error.c_str(); }
 int qd_create(const qd_problem *problem, const qd_options *options, qd_handle **out) {
   if (!out) return fail(QD_ERR_NULL, "out is NULL");
   *out = nullptr;
-  qd_handle *h = new (std::nothrow) qd_handle;
-  if (!h) return fail(QD_ERR_NOMEM, "out of host memory");
-  const int rc = create_impl(problem, options, h);
-  if (rc != QD_OK) {
-    release(h);
-    delete h;
-    return rc;
-  }
-  *out = h;
+  BatchSpec S;
+  QD_RC(spec_from_problem(problem, S));
+  return create_from_spec(S, options, out);
+}
+
+int qd_create_jk(const qd_jk_problem *problem, const qd_options *options, qd_handle **out) {
+  if (!out) return fail(QD_ERR_NULL, "out is NULL");
+  *out = nullptr;
+  BatchSpec S;
+  QD_RC(spec_from_jk(problem, S));
+  return create_from_spec(S, options, out);
+}
+
+int qd_jk_theta_grid(int32_t K, int32_t jmax, double *x, double *w) {
+  if (!x || !w) return fail(QD_ERR_NULL, "NULL argument");
+  if (K < 0 || jmax < K) return fail(QD_ERR_ARG, "need 0 <= K <= jmax");
+  std::vector<double> xv, wv, Pb;
+  qdt::assoc_legendre_dvr(K, jmax, xv, wv, Pb);
+  memcpy(x, xv.data(), sizeof(double) * xv.size());
+  memcpy(w, wv.data(), sizeof(double) * wv.size());
   return QD_OK;
 }
 
@@ -814,7 +1197,20 @@ int qd_get_crp(qd_handle *h, double *crp_out) {
   if (!h->do_crp) return fail(QD_ERR_ARG, "calculate_CRP was 0");
   QD_HIP(hipSetDevice(h->dev));
   QD_RC(compute_crp(h));
-  memcpy(crp_out, h->host_crp.data(), sizeof(double) * h->ne);
+  for (int e = 0; e < h->ne; e++) {
+    double s = 0;
+    for (int b = 0; b < h->nb; b++) s += h->host_crp[(size_t)b * h->ne + e];
+    crp_out[e] = s;
+  }
+  return QD_OK;
+}
+
+int qd_get_crp_blocks(qd_handle *h, double *crp_out) {
+  if (!h || !crp_out) return fail(QD_ERR_NULL, "NULL argument");
+  if (!h->do_crp) return fail(QD_ERR_ARG, "calculate_CRP was 0");
+  QD_HIP(hipSetDevice(h->dev));
+  QD_RC(compute_crp(h));
+  memcpy(crp_out, h->host_crp.data(), sizeof(double) * h->host_crp.size());
   return QD_OK;
 }
 
@@ -832,11 +1228,8 @@ int qd_get_steps(qd_handle *h, int32_t *steps_out) {
 int qd_get_timing(qd_handle *h, int32_t which, double *ms_out) {
   if (!h || !ms_out) return fail(QD_ERR_NULL, "NULL argument");
   if (which < 0 || which >= TK_N) return fail(QD_ERR_ARG, "bad timing selector");
-  if (which == TK_STEP) {
-    *ms_out = h->last_step_ms;
-  } else {
-    *ms_out = h->t_n[which] ? h->t_ms[which] / h->t_n[which] : 0.0;
-  }
+  if (which == TK_STEP) *ms_out = h->last_step_ms;
+  else *ms_out = h->t_n[which] ? h->t_ms[which] / (double)std::max<long long>(1, h->t_n[which]) : 0.0;
   return QD_OK;
 }
 
@@ -854,10 +1247,86 @@ int qd_time_evolution(qd_problem *problem, const qd_options *options) {
   if (rc != QD_OK) return rc;
   rc = qd_propagate(h, problem->time.total_steps);
   if (rc == QD_OK) rc = qd_get_psi(h, problem->psi);
-  if (rc == QD_OK && h->do_crp && problem->time.total_steps > 0) rc = qd_get_crp(h, problem->crp.CRP);
+  if (rc == QD_OK && h->do_crp && problem->time.total_steps > 0) {
+    if (!problem->crp.CRP) rc = fail(QD_ERR_NULL, "CRP.CRP is NULL");
+    else rc = qd_get_crp(h, problem->crp.CRP);
+  }
   if (rc == QD_OK) problem->time.steps = h->steps;
   qd_destroy(h);
   return rc;
+}
+
+}  // extern "C"
+
+// ---- state access and test hooks ----
+extern "C" {
+
+int qd_set_psi(qd_handle *h, const double *psi_in) {
+  if (!h || !psi_in) return fail(QD_ERR_NULL, "NULL argument");
+  QD_HIP(hipSetDevice(h->dev));
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMemcpy(h->psi64, psi_in, sizeof(double2) * h->npt, hipMemcpyHostToDevice));
+  } else {
+    double2 *tmp = nullptr;
+    QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(psi_in), h->npt));
+    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, tmp, h->psi32, h->npt);
+    QD_HIP(hipStreamSynchronize(h->stream));
+    QD_HIP(hipFree(tmp));
+  }
+  return QD_OK;
+}
+
+int qd_debug_get_fbr(qd_handle *h, double *out) {
+  if (!h || !out) return fail(QD_ERR_NULL, "NULL argument");
+  QD_HIP(hipSetDevice(h->dev));
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMemcpy(out, h->lpsi64, sizeof(double2) * h->nlp, hipMemcpyDeviceToHost));
+  } else {
+    double2 *tmp = nullptr;
+    QD_HIP(hipMalloc((void **)&tmp, sizeof(double2) * h->nlp));
+    hipLaunchKernelGGL(k_f32_to_f64, grid1(h->nlp), dim3(256), 0, h->stream, h->lpsi32, tmp, h->nlp);
+    QD_HIP(hipStreamSynchronize(h->stream));
+    QD_HIP(hipMemcpy(out, tmp, sizeof(double2) * h->nlp, hipMemcpyDeviceToHost));
+    QD_HIP(hipFree(tmp));
+  }
+  return QD_OK;
+}
+
+int qd_debug_set_fbr(qd_handle *h, const double *in) {
+  if (!h || !in) return fail(QD_ERR_NULL, "NULL argument");
+  QD_HIP(hipSetDevice(h->dev));
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMemcpy(h->lpsi64, in, sizeof(double2) * h->nlp, hipMemcpyHostToDevice));
+  } else {
+    double2 *tmp = nullptr;
+    QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(in), h->nlp));
+    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->nlp), dim3(256), 0, h->stream, tmp, h->lpsi32, h->nlp);
+    QD_HIP(hipStreamSynchronize(h->stream));
+    QD_HIP(hipFree(tmp));
+  }
+  return QD_OK;
+}
+
+int qd_debug_stage(qd_handle *h, int32_t stage) {
+  if (!h) return fail(QD_ERR_NULL, "handle is NULL");
+  QD_HIP(hipSetDevice(h->dev));
+  const int full = FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV;
+  int rc = QD_OK;
+  if (stage == 0) rc = launch_theta(h, true, 0, 0);
+  else if (stage == 3) rc = launch_theta(h, false, 0, 0);
+  else if (stage >= 1 && stage <= 7) {
+    const int flags = stage == 1 || stage == 2 ? full : (stage == 4 || stage == 5 ? FFT_FWD : FFT_INV);
+    const bool alongR = stage == 1 || stage == 4 || stage == 6;
+    if (h->prec == QD_PREC_FP64)
+      rc = alongR ? launch_rpass<double>(h, flags, h->lpsi64) : launch_rline<double>(h, flags, h->lpsi64);
+    else
+      rc = alongR ? launch_rpass<float>(h, flags, h->lpsi32) : launch_rline<float>(h, flags, h->lpsi32);
+  } else {
+    return fail(QD_ERR_ARG, "bad stage");
+  }
+  if (rc != QD_OK) return rc;
+  QD_HIP(hipStreamSynchronize(h->stream));
+  return QD_OK;
 }
 
 }  // extern "C"

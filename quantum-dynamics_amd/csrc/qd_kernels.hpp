@@ -3,19 +3,26 @@
 // One step of the reference (common/timeEvol.C:619-647, evolution_dt) is
 //   V phase -> theta forward (DVR->FBR) -> rot/2 -> 2-D FFT -> kinetic ->
 //   2-D IFFT -> rot/2 -> theta backward (FBR->DVR) -> dump -> flux surface.
-// Here it runs as four HBM sweeps:
-//   theta_gemm<fwd>   V phase fused into the operand load (prologue)
-//   fft_lines<R>      rot/2, FFT along R, exp(-i T1 dt)/n1, IFFT along R
-//   fft_lines<r>      FFT along r, exp(-i T2 dt)/n2, IFFT along r, rot/2
-//   theta_gemm<bwd>   dump fused into the store (epilogue)
-// plus a small flux-surface extraction kernel and, once per time block, the
-// time->energy accumulation.  The kinetic propagator factorises,
-// exp(-i(T1+T2)dt) = exp(-i T1 dt) exp(-i T2 dt) (timeEvol.C:514), so the 2-D
-// FFT splits into independent 1-D passes that each fit in LDS.
+// Here it runs as four HBM sweeps over a batch of (J, K) blocks:
+//   k_theta_*  (fwd)   V phase fused into the operand load
+//   k_rpass            K->l mixing, D_R/2, FFT_R, exp(-i T1 dt)/n1, IFFT_R, D_R/2, l->K mixing
+//   k_rline            D_r/2, FFT_r, exp(-i T2 dt)/n2, IFFT_r, D_r/2
+//   k_theta_*  (bwd)   dump fused into the store
+// plus the flux-surface extraction and, once per time block, the
+// time->energy accumulation.
 //
-// Layout (column-major, R fastest, common/mat.h:18-41): psi[i + n1 (j + n2 k)],
-// complex interleaved.  The theta transforms treat psi as a real matrix of
-// 2 n1 n2 rows and ntheta columns (W and P are real, LegTransform.F:6-59).
+// Why the middle of the step splits this way.  With D_R = exp(-i dt/2 l(l+1)/(2 mu1 R^2))
+// (l the orbital quantum number, = j for J = 0) and D_r = exp(-i dt/2 j(j+1)/(2 mu2 r^2)),
+// the reference's  rot/2 . Kin . rot/2  equals
+//   [D_r/2 Kin_r D_r/2] . U [D_R/2 Kin_R D_R/2] U^T
+// exactly, because Kin_R, Kin_r, D_R, D_r act on different axes and U (the
+// K <-> l transform of the body-fixed blocks, identity for J = 0) touches
+// neither R nor r.  So each 1-D pass carries its own phases and the K mixing
+// happens inside the R pass, on lines already in LDS.
+//
+// Layout.  Blocks sit back to back; block b owns DVR columns dcol0..dcol0+nd-1
+// and FBR columns fcol0..fcol0+nf-1.  A column is one n1 x n2 slab,
+// column-major with R fastest (common/mat.h:18-41), so psi[i + n1 j + n1 n2 c].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,7 +62,7 @@ template <typename C> __device__ __forceinline__ C csub(C a, C b) {
   r.y = a.y - b.y;
   return r;
 }
-// i * s * a for s = +-1
+// i * s * a for real s
 template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) {
   C r;
   r.x = -a.y * s;
@@ -64,19 +71,32 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
 }
 
 // ---------------------------------------------------------------------------
-// Elementwise kernels (first-step phase, plain-pipeline operators, conversion)
+// Batch description
 // ---------------------------------------------------------------------------
+struct BlockDesc {
+  int J, K;
+  int nd, nf;        // theta DVR points, FBR functions
+  int dcol0, fcol0;  // first DVR / FBR column
+  int mat;           // index of the block's theta matrices
+  int pad;
+};
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels
+// ---------------------------------------------------------------------------
+// V of point x (global DVR layout) is V[dcol_vidx[c] + (x mod n12)], c = x / n12.
 
 // psi /= sqrt_ref(exp(-i V dt)), the first-step half phase of the CPU
 // propagator (timeEvol.C:519-538).  sqrt_ref is the reference's pow(c, 0.5)
 // with arg() = asin(Im/|c|) plus quadrant fixes (complex.h:60-68,148-159); it
 // equals exp(+i V dt/2) only while |V dt| < pi, which is reproduced here.
-__global__ void k_prephase_cpu(double2 *psi, const double *V, double dt, size_t n) {
+__global__ void k_prephase_cpu(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
+                               double dt, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
+  const double v = V[dcol_vidx[x / n12] + (long long)(x % n12)];
   double s, c;
-  sincos(-dt * V[x], &s, &c);
-  // arg of (c, s), |c| = sqrt(c^2 + s^2)
+  sincos(-dt * v, &s, &c);
   const double a = sqrt(c * c + s * s);
   double ang = asin(s / a);
   if (c < 0 && s > 0) ang = M_PI - ang;
@@ -96,29 +116,33 @@ __global__ void k_prephase_cpu(double2 *psi, const double *V, double dt, size_t 
 
 // psi *= exp(+i V dt/2): the CUDA propagator's first-step phase
 // (evolutionCUDA2.cu:507, evolution_with_potential(-dt/2)).
-__global__ void k_prephase_cuda(double2 *psi, const double *V, double dt, size_t n) {
+__global__ void k_prephase_cuda(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
+                                double dt, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
+  const double v = V[dcol_vidx[x / n12] + (long long)(x % n12)];
   double s, c;
-  sincos(0.5 * dt * V[x], &s, &c);
+  sincos(0.5 * dt * v, &s, &c);
   psi[x] = cmul(psi[x], make_double2(c, s));
 }
 
-// psi *= exp(-i V dt) with V in FP64 (evolution_with_potential_dt, timeEvol.C:540-559)
+// psi *= exp(-i V dt) (evolution_with_potential_dt, timeEvol.C:540-559); FP64
+// V with the phase computed on the fly, or the FP32 phase table.
 template <typename T>
 __global__ void k_vphase(typename cx_of<T>::type *psi, const double *V, const typename cx_of<T>::type *expV,
-                         double dt, size_t n) {
+                         const long long *dcol_vidx, size_t n12, double dt, size_t n) {
   typedef typename cx_of<T>::type C;
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
+  const long long vi = dcol_vidx[x / n12] + (long long)(x % n12);
   C e;
   if (V) {
     double s, c;
-    sincos(-dt * V[x], &s, &c);
+    sincos(-dt * V[vi], &s, &c);
     e.x = (T)c;
     e.y = (T)s;
   } else {
-    e = expV[x];
+    e = expV[vi];
   }
   psi[x] = cmul(psi[x], e);
 }
@@ -135,17 +159,18 @@ __global__ void k_dump(typename cx_of<T>::type *psi, const double *d1, const dou
   psi[x] = cscale(psi[x], (T)d);
 }
 
-// lpsi *= rotR[l][i] rotr[l][j]  (evolution_with_rotational_dt_2, timeEvol.C:561-582)
+// lpsi *= D_R/2[lam(c)][i] D_r/2[jr(c)][j]  (evolution_with_rotational_dt_2, timeEvol.C:561-582)
 template <typename T>
-__global__ void k_rot(typename cx_of<T>::type *lpsi, const typename cx_of<T>::type *rotR,
-                      const typename cx_of<T>::type *rotr, int n1, int n2, size_t n) {
+__global__ void k_rot(typename cx_of<T>::type *lpsi, const typename cx_of<T>::type *dR,
+                      const typename cx_of<T>::type *dr, const int *fcol_lam, const int *fcol_j, int n1, int n2,
+                      size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const int i = (int)(x % n1);
-  const size_t jl = x / n1;
-  const int j = (int)(jl % n2);
-  const int l = (int)(jl / n2);
-  lpsi[x] = cmul(lpsi[x], cmul(rotR[(size_t)l * n1 + i], rotr[(size_t)l * n2 + j]));
+  const size_t jc = x / n1;
+  const int j = (int)(jc % n2);
+  const int c = (int)(jc / n2);
+  lpsi[x] = cmul(lpsi[x], cmul(dR[(size_t)fcol_lam[c] * n1 + i], dr[(size_t)fcol_j[c] * n2 + j]));
 }
 
 // lpsi *= kinR[i] kinr[j]  (evolution_with_kinetic_dt, timeEvol.C:584-606; the
@@ -170,25 +195,11 @@ __global__ void k_f32_to_f64(const float2 *a, double2 *b, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged Stockham FFT along one axis with fused phases
+// Stockham FFT building blocks
 // ---------------------------------------------------------------------------
-// A block transforms LPB lines of length n.  Lines along R are contiguous
-// (line id j + n2 l, element stride 1); lines along r are strided by n1
-// (line id i + n1 l, element stride n1) and a block takes LPB consecutive i so
-// every global access is coalesced across the block.  Passes follow the
-// autosort Stockham scheme: pass with radix r and span Ns reads
-// x[b + q n/r], twiddles by w^(q (b mod Ns) n/(Ns r)), and writes
-// y[(b/Ns) Ns r + (b mod Ns) + q Ns].  The first pass reads global memory,
-// the last pass writes it; the passes in between go through LDS.
-
-enum {
-  FFT_PRE_ROT = 1,   // multiply by rotR[l][i] rotr[l][j] before the forward FFT
-  FFT_FWD = 2,
-  FFT_KIN = 4,       // multiply the spectrum by kin[p]
-  FFT_INV = 8,
-  FFT_POST_ROT = 16  // multiply by the rot tables after the inverse FFT
-};
-
+// Pass with radix r and span Ns reads x[b + q n/r], twiddles by
+// w^(q (b mod Ns) n/(Ns r)), and writes y[(b/Ns) Ns r + (b mod Ns) + q Ns]
+// (autosort Stockham; any order of the radices gives the DFT).
 constexpr int FFT_MAX_PASS = 12;
 constexpr int FFT_BPT = 4;  // butterflies per thread per pass (max)
 
@@ -198,18 +209,12 @@ struct FftPlanArg {
   int radix[FFT_MAX_PASS];
 };
 
-template <typename T>
-struct FftArgs {
-  typename cx_of<T>::type *data;
-  const typename cx_of<T>::type *tw;    // tw[e] = exp(-2 pi i e / n)
-  const typename cx_of<T>::type *kin;   // [n] spectrum multiplier or null
-  const typename cx_of<T>::type *rotR;  // [m][n1]
-  const typename cx_of<T>::type *rotr;  // [m][n2]
-  int n1, n2, m;
-  int lpb;      // lines per block
-  int nlines;   // total lines
-  int flags;
-  FftPlanArg plan;
+enum {
+  FFT_FWD = 1,
+  FFT_KIN = 2,       // multiply the spectrum by kin[p]
+  FFT_INV = 4,
+  FFT_PHASE = 8,     // D/2 before the forward and after the inverse transform
+  FFT_MIX = 16       // K <-> l mixing of line groups (R pass)
 };
 
 template <typename C, typename T>
@@ -254,147 +259,363 @@ __device__ __forceinline__ void dft_small(C *v, int r, T sign) {
   }
 }
 
-template <typename T, int AXIS>
-__global__ __launch_bounds__(256) void k_fft_lines(FftArgs<T> A) {
+// One Stockham pass on the lines of a work group, in place in LDS.
+// lidx(line, p) maps a line element to LDS; `slot`/`nslots` give this thread's
+// lines and `bt`/`tpl` its butterflies.  Two barriers: all reads complete
+// before any write.
+template <typename T, typename LIDX, typename MUL>
+__device__ __forceinline__ void stockham_pass(typename cx_of<T>::type *lds, LIDX lidx, int nlines, int slot,
+                                              int nslots, int bt, int tpl, int n, int r, int Ns, bool fwd,
+                                              const typename cx_of<T>::type *tw, MUL post) {
+  typedef typename cx_of<T>::type C;
+  const T sign = fwd ? (T)-1 : (T)1;
+  const int nb = n / r;
+  C v[FFT_BPT][5];
+  const int line = slot;  // each slot owns at most one line (host guarantees nlines <= nslots)
+  const bool act = line < nlines;
+#pragma unroll
+  for (int bb = 0; bb < FFT_BPT; bb++) {
+    const int b = bt + bb * tpl;
+    if (act && b < nb) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        if (q >= r) break;
+        v[bb][q] = lds[lidx(line, b + q * nb)];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < FFT_BPT; bb++) {
+    const int b = bt + bb * tpl;
+    if (act && b < nb) {
+      const int k = b % Ns;
+      if (Ns > 1) {
+        const int estep = k * (n / (Ns * r));
+#pragma unroll
+        for (int q = 1; q < 5; q++) {
+          if (q >= r) break;
+          C w = tw[q * estep];
+          if (!fwd) w.y = -w.y;
+          v[bb][q] = cmul(v[bb][q], w);
+        }
+      }
+      dft_small<C, T>(v[bb], r, sign);
+      const int od = (b / Ns) * Ns * r + k;
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        if (q >= r) break;
+        const int p = od + q * Ns;
+        lds[lidx(line, p)] = post(line, p, v[bb][q]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// R pass: lines along R (contiguous), grouped for the K <-> l mixing
+// ---------------------------------------------------------------------------
+// A work group takes one bundle of lines.  Lines come in units: a unit is the
+// g FBR lines (J, K = 0..g-1, j) at one r index, mixed by the g x g matrix U
+// of (J, j): c_m = sum_K U[K][m] c_K before the transform and c_K = sum_m
+// U[K][m] c_m after it.  For J = 0 (and j = 0) units are single lines.
+constexpr int RP_THREADS = 512;
+constexpr int RP_GMAX = 32;  // largest unit (mixing group)
+
+struct RUnit {
+  int col0;   // index into unit_cols of the unit's first column
+  int g;      // lines in the unit
+  int u0;     // offset of U in umat (g*g, row K, column m), -1 = identity
+  int r;      // r index of the lines
+};
+
+template <typename T>
+struct RPassArgs {
+  typename cx_of<T>::type *data;
+  const typename cx_of<T>::type *tw;   // exp(-2 pi i e / n1)
+  const typename cx_of<T>::type *kin;  // exp(-i T1 dt) / n1
+  const typename cx_of<T>::type *dR;   // D_R/2 table [lam][n1]
+  const int *fcol_lam;                 // lam index per FBR column (in the l basis)
+  const int *bundle_u0;                // [nbundle + 1] range of units per bundle
+  const RUnit *units;
+  const int *unit_cols;                // FBR columns of the units
+  const T *umat;
+  int n1, n2;
+  int tpl;
+  int flags;
+  FftPlanArg plan;
+};
+
+template <typename T>
+__global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
   typedef typename cx_of<T>::type C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
-  const int n = A.plan.n;
+  __shared__ long long s_off[64];   // global offset of each line of the bundle
+  __shared__ int s_lam[64];
+  __shared__ int s_unit_line0[65];
+  const int n = A.n1;
+  const int t = threadIdx.x;
+  const int ub = A.bundle_u0[blockIdx.x], ue = A.bundle_u0[blockIdx.x + 1];
+  const int nu = ue - ub;
+  // line table of the bundle (nu <= 64 units, lines <= 64)
+  if (t == 0) {
+    int L = 0;
+    for (int u = 0; u < nu; u++) {
+      s_unit_line0[u] = L;
+      L += A.units[ub + u].g;
+    }
+    s_unit_line0[nu] = L;
+  }
+  __syncthreads();
+  const int nl = s_unit_line0[nu];
+  if (t < nl) {
+    int u = 0;
+    while (s_unit_line0[u + 1] <= t) u++;
+    const RUnit U = A.units[ub + u];
+    const int col = A.unit_cols[U.col0 + (t - s_unit_line0[u])];
+    s_off[t] = ((long long)col * A.n2 + U.r) * n;
+    s_lam[t] = A.fcol_lam[col];
+  }
+  __syncthreads();
+  // load
+  for (int e = t; e < nl * n; e += RP_THREADS) {
+    const int L = e / n, p = e - (e / n) * n;
+    lds[L * n + p] = A.data[s_off[L] + p];
+  }
+  __syncthreads();
+  const bool mix = A.flags & FFT_MIX;
+  const bool phase = A.flags & FFT_PHASE;
+  // K -> l mixing and D_R/2 (one thread per unit position)
+  if (mix || phase) {
+    for (int e = t; e < nu * n; e += RP_THREADS) {
+      const int u = e / n, p = e - (e / n) * n;
+      const RUnit U = A.units[ub + u];
+      const int L0 = s_unit_line0[u];
+      if (mix && U.u0 >= 0) {
+        C in[RP_GMAX];
+#pragma unroll
+        for (int K = 0; K < RP_GMAX; K++)
+          if (K < U.g) in[K] = lds[(L0 + K) * n + p];
+#pragma unroll 1
+        for (int m = 0; m < U.g; m++) {
+          C acc;
+          acc.x = 0;
+          acc.y = 0;
+#pragma unroll
+          for (int K = 0; K < RP_GMAX; K++)
+            if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
+          if (phase) acc = cmul(acc, A.dR[(size_t)s_lam[L0 + m] * n + p]);
+          lds[(L0 + m) * n + p] = acc;
+        }
+      } else if (phase) {
+        for (int m = 0; m < U.g; m++)
+          lds[(L0 + m) * n + p] = cmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p]);
+      }
+    }
+    __syncthreads();
+  }
+  // transforms: thread -> (slot = line, butterfly lane)
+  const int tpl = A.tpl;
+  const int nslots = RP_THREADS / tpl;
+  const int slot = t / tpl, bt = t % tpl;
+  auto lidx = [n](int line, int p) { return line * n + p; };
+  auto keep = [](int, int, C x) { return x; };
+  const bool fwd = A.flags & FFT_FWD, inv = A.flags & FFT_INV;
+  const int np = A.plan.npass;
+  if (fwd) {
+    int Ns = 1;
+    for (int pi = 0; pi < np; pi++) {
+      const int r = A.plan.radix[pi];
+      if (pi == np - 1 && (A.flags & FFT_KIN)) {
+        auto kin = [&](int, int p, C x) { return cmul(x, A.kin[p]); };
+        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, kin);
+      } else {
+        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, keep);
+      }
+      Ns *= r;
+    }
+  }
+  if (inv) {
+    int Ns = 1;
+    for (int pi = 0; pi < np; pi++) {
+      const int r = A.plan.radix[pi];
+      stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, false, A.tw, keep);
+      Ns *= r;
+    }
+  }
+  // D_R/2 and l -> K mixing
+  if (mix || phase) {
+    for (int e = t; e < nu * n; e += RP_THREADS) {
+      const int u = e / n, p = e - (e / n) * n;
+      const RUnit U = A.units[ub + u];
+      const int L0 = s_unit_line0[u];
+      if (mix && U.u0 >= 0) {
+        C in[RP_GMAX];
+#pragma unroll
+        for (int m = 0; m < RP_GMAX; m++)
+          if (m < U.g) {
+            C x = lds[(L0 + m) * n + p];
+            if (phase) x = cmul(x, A.dR[(size_t)s_lam[L0 + m] * n + p]);
+            in[m] = x;
+          }
+#pragma unroll 1
+        for (int K = 0; K < U.g; K++) {
+          C acc;
+          acc.x = 0;
+          acc.y = 0;
+#pragma unroll
+          for (int m = 0; m < RP_GMAX; m++)
+            if (m < U.g) acc = cadd(acc, cscale(in[m], A.umat[U.u0 + K * U.g + m]));
+          lds[(L0 + K) * n + p] = acc;
+        }
+      } else if (phase) {
+        for (int m = 0; m < U.g; m++)
+          lds[(L0 + m) * n + p] = cmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p]);
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nl * n; e += RP_THREADS) {
+    const int L = e / n, p = e - (e / n) * n;
+    A.data[s_off[L] + p] = lds[L * n + p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// r pass: lines along r (stride n1); a work group takes LPB consecutive i of
+// one FBR column so global accesses are coalesced across the group
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RLineArgs {
+  typename cx_of<T>::type *data;
+  const typename cx_of<T>::type *tw;   // exp(-2 pi i e / n2)
+  const typename cx_of<T>::type *kin;  // exp(-i T2 dt) / n2
+  const typename cx_of<T>::type *dr;   // D_r/2 table [j][n2]
+  const int *fcol_j;
+  int n1, n2;
+  int lpb;
+  int nlines;  // n1 * FBR columns
+  int flags;
+  FftPlanArg plan;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
+  typedef typename cx_of<T>::type C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *lds = reinterpret_cast<C *>(smem_raw);
+  const int n = A.n2;
   const int lpb = A.lpb;
   const int tpl = 256 / lpb;
   const int t = threadIdx.x;
-  const int line_local = AXIS == 0 ? t / tpl : t % lpb;
-  const int bt = AXIS == 0 ? t % tpl : t / lpb;
+  const int line_local = t % lpb;
+  const int bt = t / lpb;
   const int line = blockIdx.x * lpb + line_local;
   const bool active = line < A.nlines;
-  const size_t n12 = (size_t)A.n1 * A.n2;
-  // line coordinates
-  int jl_fixed;  // R axis: j of the line; r axis: i of the line
-  int l;
-  size_t base;
-  size_t stride;
-  if (AXIS == 0) {
-    const int lc = active ? line : 0;
-    jl_fixed = lc % A.n2;
-    l = lc / A.n2;
-    base = (size_t)lc * A.n1;
-    stride = 1;
-  } else {
-    const int lc = active ? line : 0;
-    jl_fixed = lc % A.n1;
-    l = lc / A.n1;
-    base = (size_t)l * n12 + jl_fixed;
-    stride = A.n1;
+  const int lc = active ? line : 0;
+  const int i = lc % A.n1;
+  const int c = lc / A.n1;
+  const size_t base = (size_t)c * A.n1 * A.n2 + i;
+  const size_t stride = A.n1;
+  const int jphase = A.fcol_j[c];
+  auto lidx = [lpb, line_local](int, int p) { return p * lpb + line_local; };
+  // load (the element order p*lpb + line_local keeps the group coalesced)
+  for (int p = bt; p < n; p += tpl) {
+    C x;
+    x.x = 0;
+    x.y = 0;
+    if (active) {
+      x = A.data[base + (size_t)p * stride];
+      if (A.flags & FFT_PHASE) x = cmul(x, A.dr[(size_t)jphase * n + p]);
+    }
+    lds[p * lpb + line_local] = x;
   }
-  auto lidx = [&](int p) { return AXIS == 0 ? line_local * n + p : p * lpb + line_local; };
-
-  const bool fwd = A.flags & FFT_FWD, inv = A.flags & FFT_INV;
-  const int npass = A.plan.npass;
-  const int ptot = (fwd ? npass : 0) + (inv ? npass : 0);
-
-  C v[FFT_BPT][5];
-  for (int pass = 0; pass < ptot; pass++) {
-    const bool is_fwd_pass = fwd && pass < npass;
-    const int pi = is_fwd_pass ? pass : (fwd ? pass - npass : pass);
-    const int r = A.plan.radix[pi];
+  __syncthreads();
+  auto keep = [](int, int, C x) { return x; };
+  const int np = A.plan.npass;
+  // each "slot" is a line; one line per thread group of tpl
+  if (A.flags & FFT_FWD) {
     int Ns = 1;
-    for (int q = 0; q < pi; q++) Ns *= A.plan.radix[q];
-    const T sign = is_fwd_pass ? (T)-1 : (T)1;
-    const int nb = n / r;
-    const int stride_q = nb;
-    // ---- read ----
-#pragma unroll
-    for (int bb = 0; bb < FFT_BPT; bb++) {
-      const int b = bt + bb * tpl;
-      if (b < nb && active) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) {
-          if (q >= r) break;
-          const int p = b + q * stride_q;
-          C x;
-          if (pass == 0) {
-            x = A.data[base + (size_t)p * stride];
-            if ((A.flags & FFT_PRE_ROT) && is_fwd_pass) {
-              const int i = AXIS == 0 ? p : jl_fixed;
-              const int j = AXIS == 0 ? jl_fixed : p;
-              x = cmul(x, cmul(A.rotR[(size_t)l * A.n1 + i], A.rotr[(size_t)l * A.n2 + j]));
-            }
-          } else {
-            x = lds[lidx(p)];
-          }
-          v[bb][q] = x;
-        }
+    for (int pi = 0; pi < np; pi++) {
+      const int r = A.plan.radix[pi];
+      if (pi == np - 1 && (A.flags & FFT_KIN)) {
+        auto kin = [&](int, int p, C x) { return cmul(x, A.kin[p]); };
+        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, kin);
+      } else {
+        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, keep);
       }
+      Ns *= r;
     }
-    if (pass > 0) __syncthreads();
-    // ---- compute + write ----
-    const bool last = pass == ptot - 1;
-    const bool last_fwd = is_fwd_pass && pass == npass - 1;
-#pragma unroll
-    for (int bb = 0; bb < FFT_BPT; bb++) {
-      const int b = bt + bb * tpl;
-      if (b < nb && active) {
-        const int k = b % Ns;
-        if (Ns > 1) {
-          const int estep = k * (n / (Ns * r));
-#pragma unroll
-          for (int q = 1; q < 5; q++) {
-            if (q >= r) break;
-            C w = A.tw[q * estep];
-            if (!is_fwd_pass) w.y = -w.y;
-            v[bb][q] = cmul(v[bb][q], w);
-          }
-        }
-        dft_small<C, T>(v[bb], r, sign);
-        const int od = (b / Ns) * Ns * r + k;
-#pragma unroll
-        for (int q = 0; q < 5; q++) {
-          if (q >= r) break;
-          const int p = od + q * Ns;
-          C x = v[bb][q];
-          if (last_fwd && (A.flags & FFT_KIN)) x = cmul(x, A.kin[p]);
-          if (last) {
-            if ((A.flags & FFT_POST_ROT) && !is_fwd_pass) {
-              const int i = AXIS == 0 ? p : jl_fixed;
-              const int j = AXIS == 0 ? jl_fixed : p;
-              x = cmul(x, cmul(A.rotR[(size_t)l * A.n1 + i], A.rotr[(size_t)l * A.n2 + j]));
-            }
-            A.data[base + (size_t)p * stride] = x;
-          } else {
-            lds[lidx(p)] = x;
-          }
-        }
-      }
+  }
+  if (A.flags & FFT_INV) {
+    int Ns = 1;
+    for (int pi = 0; pi < np; pi++) {
+      const int r = A.plan.radix[pi];
+      stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, false, A.tw, keep);
+      Ns *= r;
     }
-    if (!last) __syncthreads();
+  }
+  for (int p = bt; p < n; p += tpl) {
+    if (active) {
+      C x = lds[p * lpb + line_local];
+      if (A.flags & FFT_PHASE) x = cmul(x, A.dr[(size_t)jphase * n + p]);
+      A.data[base + (size_t)p * stride] = x;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // theta transform, FP64 on MFMA (v_mfma_f64_16x16x4f64)
 // ---------------------------------------------------------------------------
-// C[2R x N] = A[2R x K] * B[K x N] on psi viewed as a real matrix.
-// Block: 64 complex rows (128 real rows) x 128 columns, 4 waves of 32 real
-// rows each, K in chunks of 16.  B is pre-packed row-major [Kpad][Npad].
-// Prologue (PRO): 1 = multiply A by exp(-i V dt); 2 = by dump * exp(-i V dt).
-// Epilogue (EPI): 1 = multiply the result by the dump d1[i] d2[j].
-constexpr int TG_ROWS = 64;    // complex rows per block
-constexpr int TG_COLS = 128;   // output columns per block
+// Per block: C[2R x N] = A[2R x K] * B[K x N] on psi viewed as a real matrix
+// (W and P are real, LegTransform.F:6-59).  Work group: 64 complex rows
+// (128 real rows) x 128 columns of one block, 4 waves of 32 real rows, K in
+// chunks of 16.  B is packed per matrix as [Kpad][Npad] (row-major).
+// Prologue PRO: 1 = multiply A by exp(-i V dt); 2 = by dump * exp(-i V dt).
+// Epilogue EPI: 1 = multiply the result by the dump d1[i] d2[j].
+constexpr int TG_ROWS = 64;    // complex rows per work group
+constexpr int TG_COLS = 128;   // output columns per work group
 constexpr int TG_KC64 = 16;    // K chunk for FP64
 constexpr int TG_LDS64 = 144;  // padded row length (doubles) of the LDS tiles
 
+struct ThetaArgs {
+  const void *A;
+  void *C;
+  const void *B;               // packed matrices, mat_stride elements apart
+  long long mat_stride;
+  int Kpad, Npad;
+  const BlockDesc *blocks;
+  int fwd;                     // 1: DVR -> FBR, 0: FBR -> DVR
+  int R;                       // n1 * n2
+  const double *V;             // FP64 potential (FP64 path)
+  const void *expV;            // FP32 phase table (mixed path)
+  const long long *dcol_vidx;  // V column start per DVR column
+  double dt;
+  const double *d1, *d2;
+  const float *d1f, *d2f;
+  int n1;
+};
+
 template <int PRO, int EPI>
-__global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A, double2 *__restrict__ Cm,
-                                                   const double *__restrict__ Bp, int R, int K, int N,
-                                                   int Npad, const double *__restrict__ V, double dt,
-                                                   const double *__restrict__ d1, const double *__restrict__ d2,
-                                                   int n1) {
+__global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
   __shared__ __align__(16) double As[TG_KC64][TG_LDS64];
   __shared__ __align__(16) double Bs[TG_KC64][TG_LDS64];
+  const BlockDesc bd = P.blocks[blockIdx.z];
+  const int R = P.R;
+  const int K = P.fwd ? bd.nd : bd.nf;
+  const int N = P.fwd ? bd.nf : bd.nd;
+  const size_t acol0 = P.fwd ? bd.dcol0 : bd.fcol0;
+  const size_t ccol0 = P.fwd ? bd.fcol0 : bd.dcol0;
+  const double2 *A = reinterpret_cast<const double2 *>(P.A) + acol0 * R;
+  double2 *Cm = reinterpret_cast<double2 *>(P.C) + ccol0 * R;
+  const double *Bp = reinterpret_cast<const double *>(P.B) + (size_t)bd.mat * P.mat_stride;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int row0 = blockIdx.x * TG_ROWS;
   const int col0 = blockIdx.y * TG_COLS;
+  if (col0 >= N) return;
   d4v acc[2][8];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -402,7 +623,6 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
     for (int b = 0; b < 8; b++) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
 
   for (int k0 = 0; k0 < K; k0 += TG_KC64) {
-    // ---- A chunk: 64 complex rows x 16 k, phase prologue ----
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int cr = t & 63;
@@ -411,14 +631,13 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
       const int k = k0 + kk;
       double2 v = make_double2(0.0, 0.0);
       if (r < R && k < K) {
-        const size_t x = (size_t)r + (size_t)R * k;
-        v = A[x];
+        v = A[(size_t)r + (size_t)R * k];
         if (PRO) {
           double sn, cs;
-          sincos(-dt * V[x], &sn, &cs);
+          sincos(-P.dt * P.V[P.dcol_vidx[acol0 + k] + r], &sn, &cs);
           v = cmul(v, make_double2(cs, sn));
           if (PRO == 2) {
-            const double d = d1[r % n1] * d2[r / n1];
+            const double d = P.d1[r % P.n1] * P.d2[r / P.n1];
             v.x *= d;
             v.y *= d;
           }
@@ -426,20 +645,23 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
       }
       *reinterpret_cast<double2 *>(&As[kk][2 * cr]) = v;
     }
-    // ---- B chunk: 16 k x 128 columns ----
 #pragma unroll
     for (int s = 0; s < 8; s++) {
       const int col = t & 127;
       const int kk = (t >> 7) + 2 * s;
-      Bs[kk][col] = Bp[(size_t)(k0 + kk) * Npad + col0 + col];
+      Bs[kk][col] = Bp[(size_t)(k0 + kk) * P.Npad + col0 + col];
     }
     __syncthreads();
+    // v_mfma_f64_16x16x4f64 returns D[(lane/16) + 4v][lane%16] in register v.
+    // Feeding MFMA row i from tile row 4(i%4) + i/4 makes register v of lane l
+    // hold tile row 4(l/16) + v, so each lane ends with two whole complex rows.
+    const int arow = 4 * (lane & 3) + ((lane & 15) >> 2);
 #pragma unroll
     for (int ks = 0; ks < TG_KC64 / 4; ks++) {
       const int kr = ks * 4 + (lane >> 4);
       double a[2];
 #pragma unroll
-      for (int rb = 0; rb < 2; rb++) a[rb] = As[kr][wave * 32 + rb * 16 + (lane & 15)];
+      for (int rb = 0; rb < 2; rb++) a[rb] = As[kr][wave * 32 + rb * 16 + arow];
 #pragma unroll
       for (int cb = 0; cb < 8; cb++) {
         const double b = Bs[kr][cb * 16 + (lane & 15)];
@@ -449,11 +671,11 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
     }
     __syncthreads();
   }
-  // ---- epilogue: lane holds rows 4(lane/16)+0..3 of column lane%16 ----
+  // lane l holds tile rows 4(l/16)+0..3 (complex rows 2(l/16), 2(l/16)+1) of column l%16
 #pragma unroll
   for (int rb = 0; rb < 2; rb++) {
-    const int rr = wave * 32 + rb * 16 + 4 * (lane >> 4);  // real row, multiple of 4
-    const int r = row0 + rr / 2;                           // first of two complex rows
+    const int rr = wave * 32 + rb * 16 + 4 * (lane >> 4);
+    const int r = row0 + rr / 2;
 #pragma unroll
     for (int cb = 0; cb < 8; cb++) {
       const int col = col0 + cb * 16 + (lane & 15);
@@ -462,23 +684,19 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
       double2 c1 = make_double2(acc[rb][cb][2], acc[rb][cb][3]);
       if (EPI) {
         if (r < R) {
-          const double d = d1[r % n1] * d2[r / n1];
+          const double d = P.d1[r % P.n1] * P.d2[r / P.n1];
           c0.x *= d;
           c0.y *= d;
         }
         if (r + 1 < R) {
-          const double d = d1[(r + 1) % n1] * d2[(r + 1) / n1];
+          const double d = P.d1[(r + 1) % P.n1] * P.d2[(r + 1) / P.n1];
           c1.x *= d;
           c1.y *= d;
         }
       }
       const size_t x = (size_t)r + (size_t)R * col;
-      if (r + 1 < R) {
-        Cm[x] = c0;
-        Cm[x + 1] = c1;
-      } else if (r < R) {
-        Cm[x] = c0;
-      }
+      if (r < R) Cm[x] = c0;
+      if (r + 1 < R) Cm[x + 1] = c1;
     }
   }
 }
@@ -489,33 +707,40 @@ __global__ __launch_bounds__(256) void k_theta_f64(const double2 *__restrict__ A
 // Each FP32 operand is split into NS bf16 pieces a = a0 + a1 (+ a2); the
 // products a_p b_q with p + q <= NS - 1 are accumulated in FP32 (NS = 2: the
 // hi*hi + hi*lo + lo*hi scheme; NS = 3: six products, FP32-level accuracy).
-// B is pre-split on the host as Bs[piece][Npad][Kpad] (k contiguous).
-// Block: 64 complex rows x 128 columns; K in chunks of 32.
+// B is pre-split per matrix as [piece][Npad][Kpad] (k contiguous).
+// NCB = 16-column tiles per work group (8: 128 columns, 4: 64 columns).
 constexpr int TG_KC16 = 32;
 constexpr int TG_LDS16 = 40;  // padded k length (bf16) of an LDS row
 
 __device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }
 
-template <int NS, int PRO, int EPI>
-__global__ __launch_bounds__(256) void k_theta_bf16(const float2 *__restrict__ A, float2 *__restrict__ Cm,
-                                                    const __bf16 *__restrict__ Bsplit, int R, int K, int N,
-                                                    int Npad, int Kpad, const float2 *__restrict__ expV,
-                                                    const float *__restrict__ d1, const float *__restrict__ d2,
-                                                    int n1) {
+template <int NS, int NCB, int PRO, int EPI>
+__global__ __launch_bounds__(256) void k_theta_bf16(ThetaArgs P) {
   __shared__ __align__(16) __bf16 As[NS][128][TG_LDS16];
-  __shared__ __align__(16) __bf16 Bs[NS][TG_COLS][TG_LDS16];
+  __shared__ __align__(16) __bf16 Bs[NS][NCB * 16][TG_LDS16];
+  const BlockDesc bd = P.blocks[blockIdx.z];
+  const int R = P.R;
+  const int K = P.fwd ? bd.nd : bd.nf;
+  const int N = P.fwd ? bd.nf : bd.nd;
+  const size_t acol0 = P.fwd ? bd.dcol0 : bd.fcol0;
+  const size_t ccol0 = P.fwd ? bd.fcol0 : bd.dcol0;
+  const float2 *A = reinterpret_cast<const float2 *>(P.A) + acol0 * R;
+  float2 *Cm = reinterpret_cast<float2 *>(P.C) + ccol0 * R;
+  const __bf16 *Bsplit = reinterpret_cast<const __bf16 *>(P.B) + (size_t)bd.mat * P.mat_stride;
+  const float2 *expV = reinterpret_cast<const float2 *>(P.expV);
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int row0 = blockIdx.x * TG_ROWS;
-  const int col0 = blockIdx.y * TG_COLS;
-  f4v acc[2][8];
+  const int col0 = blockIdx.y * NCB * 16;
+  if (col0 >= N) return;
+  f4v acc[2][NCB];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 8; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NCB; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 
   for (int k0 = 0; k0 < K; k0 += TG_KC16) {
-    // ---- A chunk: thread owns complex row cr and 8 consecutive k ----
+    // A chunk: thread owns complex row cr and 8 consecutive k
     {
       const int cr = t & 63;
       const int kg = t >> 6;  // 0..3
@@ -526,12 +751,11 @@ __global__ __launch_bounds__(256) void k_theta_bf16(const float2 *__restrict__ A
         const int k = k0 + kg * 8 + e;
         float2 v = make_float2(0.f, 0.f);
         if (r < R && k < K) {
-          const size_t x = (size_t)r + (size_t)R * k;
-          v = A[x];
+          v = A[(size_t)r + (size_t)R * k];
           if (PRO) {
-            v = cmul(v, expV[x]);
+            v = cmul(v, expV[P.dcol_vidx[acol0 + k] + r]);
             if (PRO == 2) {
-              const float d = d1[r % n1] * d2[r / n1];
+              const float d = P.d1f[r % P.n1] * P.d2f[r / P.n1];
               v.x *= d;
               v.y *= d;
             }
@@ -556,18 +780,17 @@ __global__ __launch_bounds__(256) void k_theta_bf16(const float2 *__restrict__ A
         *reinterpret_cast<bf8v *>(&As[p][2 * cr + 1][kg * 8]) = hi;
       }
     }
-    // ---- B chunk: 128 columns x 32 k per piece ----
-    {
-      const int col = t >> 1;
-      const int half = t & 1;
-#pragma unroll
-      for (int p = 0; p < NS; p++) {
-        const __bf16 *src = Bsplit + ((size_t)p * Npad + col0 + col) * Kpad + k0 + half * 16;
-        const bf8v b0 = *reinterpret_cast<const bf8v *>(src);
-        const bf8v b1 = *reinterpret_cast<const bf8v *>(src + 8);
-        *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16]) = b0;
-        *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16 + 8]) = b1;
-      }
+    // B chunk: NCB*16 columns x 32 k per piece
+    for (int u = t; u < NS * NCB * 16 * 2; u += 256) {
+      const int p = u / (NCB * 32);
+      const int rem = u % (NCB * 32);
+      const int col = rem >> 1;
+      const int half = rem & 1;
+      const __bf16 *src = Bsplit + ((size_t)p * P.Npad + col0 + col) * P.Kpad + k0 + half * 16;
+      const bf8v b0 = *reinterpret_cast<const bf8v *>(src);
+      const bf8v b1 = *reinterpret_cast<const bf8v *>(src + 8);
+      *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16]) = b0;
+      *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16 + 8]) = b1;
     }
     __syncthreads();
     bf8v a[NS][2];
@@ -577,7 +800,7 @@ __global__ __launch_bounds__(256) void k_theta_bf16(const float2 *__restrict__ A
       for (int rb = 0; rb < 2; rb++)
         a[p][rb] = *reinterpret_cast<const bf8v *>(&As[p][wave * 32 + rb * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
-    for (int cb = 0; cb < 8; cb++) {
+    for (int cb = 0; cb < NCB; cb++) {
       bf8v b[NS];
 #pragma unroll
       for (int q = 0; q < NS; q++)
@@ -600,37 +823,33 @@ __global__ __launch_bounds__(256) void k_theta_bf16(const float2 *__restrict__ A
     const int rr = wave * 32 + rb * 16 + 4 * (lane >> 4);
     const int r = row0 + rr / 2;
 #pragma unroll
-    for (int cb = 0; cb < 8; cb++) {
+    for (int cb = 0; cb < NCB; cb++) {
       const int col = col0 + cb * 16 + (lane & 15);
       if (col >= N) continue;
       float2 c0 = make_float2(acc[rb][cb][0], acc[rb][cb][1]);
       float2 c1 = make_float2(acc[rb][cb][2], acc[rb][cb][3]);
       if (EPI) {
         if (r < R) {
-          const float d = d1[r % n1] * d2[r / n1];
+          const float d = P.d1f[r % P.n1] * P.d2f[r / P.n1];
           c0.x *= d;
           c0.y *= d;
         }
         if (r + 1 < R) {
-          const float d = d1[(r + 1) % n1] * d2[(r + 1) / n1];
+          const float d = P.d1f[(r + 1) % P.n1] * P.d2f[(r + 1) / P.n1];
           c1.x *= d;
           c1.y *= d;
         }
       }
       const size_t x = (size_t)r + (size_t)R * col;
-      if (r + 1 < R) {
-        Cm[x] = c0;
-        Cm[x + 1] = c1;
-      } else if (r < R) {
-        Cm[x] = c0;
-      }
+      if (r < R) Cm[x] = c0;
+      if (r + 1 < R) Cm[x + 1] = c1;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 // Flux surface: psi_s and d psi_s / dr on r = r_s (Gradient3.F:9-167,
-// gradients2.cu:583-610), written into slot `slot` of the time block.
+// gradients2.cu:583-610) for every DVR column, into slot `slot` of the time block.
 // ---------------------------------------------------------------------------
 struct StencilArg {
   int h;
@@ -640,10 +859,10 @@ struct StencilArg {
 
 template <typename T>
 __global__ void k_flux_extract(const typename cx_of<T>::type *__restrict__ psi, double2 *__restrict__ bpsi,
-                               double2 *__restrict__ bdpsi, int n1, int n2, int ntheta, int s, double dr,
+                               double2 *__restrict__ bdpsi, int n1, int n2, int ncols, int s, double dr,
                                StencilArg st, int slot) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nx = n1 * ntheta;
+  const int nx = n1 * ncols;
   if (x >= nx) return;
   const int i = x % n1, k = x / n1;
   const size_t base = (size_t)i + (size_t)n1 * n2 * k;
@@ -667,7 +886,7 @@ __global__ void k_flux_extract(const typename cx_of<T>::type *__restrict__ psi, 
 // the block):  fai[x, e] += sum_t tau[t, e] psi_s[t, x],  same for d fai,
 // tau[t, e] = dt exp(i E_e (n0 + t + 1) dt)  (the reference's recursion
 // timeEvol.C:799-817 evaluated directly, as evolutionCUDA2.cu:143-145 does).
-// Block: 64 x-values x 32 energies; each thread holds 8 energies.
+// Work group: 64 x-values x 32 energies; each thread holds 8 energies.
 // ---------------------------------------------------------------------------
 constexpr int FX_X = 64;
 constexpr int FX_E = 32;
@@ -684,7 +903,7 @@ __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restri
   __shared__ double2 stau[FX_TC][FX_E];
   const int t = threadIdx.x;
   const int xl = t & 63;
-  const int eg = t >> 6;  // 0..3 -> energies eg*8 .. eg*8+7 of the tile
+  const int eg = t >> 6;
   const int x = blockIdx.x * FX_X + xl;
   const int e0 = blockIdx.y * FX_E;
   double2 af[FX_EPT], adf[FX_EPT];
@@ -742,19 +961,21 @@ __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restri
   }
 }
 
-// P(E) = Im(sum_k w_k sum_i conj(fai) dfai) / eta^2 * dr1/mu2  (CRPAux.F:34-66);
-// sign = -1 reproduces the CUDA propagator's Zdotc(dfai, fai) (evolutionCUDA2.cu:972-977).
+// P(E) of block b: Im(sum_k w_k sum_i conj(fai) dfai) / eta^2 * dr1/mu2
+// (CRPAux.F:34-66); sign = -1 reproduces the CUDA propagator's
+// Zdotc(dfai, fai) (evolutionCUDA2.cu:972-977).  Grid (ne, nblocks).
 __global__ __launch_bounds__(256) void k_crp(const double2 *__restrict__ fai, const double2 *__restrict__ dfai,
-                                             const double *__restrict__ w, const double *__restrict__ eta_sq,
-                                             double *__restrict__ crp, int n1, int ntheta, double dr1_mu2,
-                                             double sign) {
+                                             const double *__restrict__ dcol_w, const double *__restrict__ eta_sq,
+                                             const BlockDesc *__restrict__ blocks, double *__restrict__ crp,
+                                             int n1, int nx, int ne, double dr1_mu2, double sign) {
   __shared__ double red[256];
   const int e = blockIdx.x;
-  const int nx = n1 * ntheta;
+  const BlockDesc bd = blocks[blockIdx.y];
+  const int x0 = n1 * bd.dcol0, x1 = n1 * (bd.dcol0 + bd.nd);
   double s = 0.0;
-  for (int x = threadIdx.x; x < nx; x += 256) {
+  for (int x = x0 + threadIdx.x; x < x1; x += 256) {
     const double2 a = fai[(size_t)e * nx + x], b = dfai[(size_t)e * nx + x];
-    s += w[x / n1] * (a.x * b.y - a.y * b.x);  // Im(conj(a) b)
+    s += dcol_w[x / n1] * (a.x * b.y - a.y * b.x);  // Im(conj(a) b)
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -762,19 +983,19 @@ __global__ __launch_bounds__(256) void k_crp(const double2 *__restrict__ fai, co
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) crp[e] = sign * red[0] / eta_sq[e] * dr1_mu2;
+  if (threadIdx.x == 0) crp[(size_t)blockIdx.y * ne + e] = sign * red[0] / eta_sq[e] * dr1_mu2;
 }
 
-// per-block partial sums of w_k |psi|^2 (module_for_psi, timeEvol.C:220-243)
+// per-group partial sums of w_k |psi|^2 (module_for_psi, timeEvol.C:220-243)
 template <typename T>
 __global__ __launch_bounds__(256) void k_norm_partial(const typename cx_of<T>::type *__restrict__ psi,
-                                                      const double *__restrict__ w, size_t n12, size_t n,
+                                                      const double *__restrict__ dcol_w, size_t n12, size_t n,
                                                       double *__restrict__ out) {
   __shared__ double red[256];
   double s = 0.0;
   for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (size_t)gridDim.x * 256) {
     const typename cx_of<T>::type v = psi[x];
-    s += w[x / n12] * ((double)v.x * v.x + (double)v.y * v.y);
+    s += dcol_w[x / n12] * ((double)v.x * v.x + (double)v.y * v.y);
   }
   red[threadIdx.x] = s;
   __syncthreads();

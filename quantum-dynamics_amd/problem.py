@@ -372,3 +372,173 @@ def config(cfg: int, **overrides) -> Problem:
         raise ValueError("unknown config %r" % (cfg,))
     kw.update(overrides)
     return make_problem(system, **kw)
+
+
+# --------------------------------------------------------------------------
+# J > 0: body-fixed K blocks with Coriolis coupling (no reference counterpart;
+# the reference propagates J = 0 only, common/timeEvol.C:487)
+# --------------------------------------------------------------------------
+def assoc_legendre_dvr(K: int, jmax: int):
+    """DVR of the normalised associated Legendre functions Pt_j^K(x), j = K..jmax.
+
+    The points x_k are the eigenvalues of x = cos(theta) in that basis
+    (tridiagonal, <jK|x|j+1 K> = sqrt(((j+1)^2 - K^2) / ((2j+1)(2j+3)))); they
+    are the Gauss-Jacobi(K, K) nodes.  With omega_k the matching weights,
+      Pb[j, k] = Pt_j^K(x_k),   Wf[k, j] = omega_k Pt_j^K(x_k),   Pb @ Wf = I.
+    For K = 0 these are the Gauss-Legendre nodes and weights.
+    Returns (x, omega, Pb) with Pb of shape (n, n), n = jmax - K + 1."""
+    n = jmax - K + 1
+    j = np.arange(K, jmax, dtype=np.float64)
+    off = np.sqrt(((j + 1) ** 2 - K * K) / ((2 * j + 1) * (2 * j + 3)))
+    X = np.diag(off, 1) + np.diag(off, -1)
+    x, U = np.linalg.eigh(X)
+    # Pt_K^K(x) = c_K (1 - x^2)^(K/2),  c_K = sqrt((2K+1)/2) sqrt((2K)!) / (2^K K!)
+    logc = 0.5 * math.log((2 * K + 1) / 2.0) + 0.5 * math.lgamma(2 * K + 1) - K * math.log(2.0) - math.lgamma(K + 1)
+    pkk = np.exp(logc) * (1 - x * x) ** (K / 2.0)
+    sgn = np.sign(U[0, :] * pkk)
+    U = U * sgn[None, :]
+    omega = (U[0, :] / pkk) ** 2
+    Pb = U / np.sqrt(omega)[None, :]
+    return x, omega, Pb
+
+
+def coriolis_matrix(J: int, j: int) -> np.ndarray:
+    """(J - j)^2 in the parity-adapted body-fixed basis K = 0..min(J, j):
+      diagonal  J(J+1) + j(j+1) - 2K^2,
+      K, K+1    -lambda+(J,K) lambda+(j,K) sqrt(1 + delta_K0),  lambda+(a,K) = sqrt(a(a+1) - K(K+1)).
+    Its eigenvalues are l(l+1), l = |J-j|..J+j with J + j + l even."""
+    nk = min(J, j) + 1
+    B = np.zeros((nk, nk))
+    for K in range(nk):
+        B[K, K] = J * (J + 1) + j * (j + 1) - 2 * K * K
+        if K + 1 < nk:
+            c = -math.sqrt(J * (J + 1) - K * (K + 1)) * math.sqrt(j * (j + 1) - K * (K + 1))
+            if K == 0:
+                c *= math.sqrt(2.0)
+            B[K, K + 1] = B[K + 1, K] = c
+    return B
+
+
+def coriolis_eigen(J: int, j: int):
+    """Eigenvectors U (columns) and the exact orbital quantum numbers l of
+    coriolis_matrix(J, j), ascending."""
+    B = coriolis_matrix(J, j)
+    lam, U = np.linalg.eigh(B)
+    l = np.rint((-1 + np.sqrt(1 + 4 * np.maximum(lam, 0))) / 2).astype(int)
+    return U, l
+
+
+@dataclasses.dataclass
+class JKBlock:
+    J: int
+    K: int
+    n: int                 # theta DVR points = FBR functions (j = K..jmax)
+    offset: int            # first complex element of the block in the batched psi
+
+
+@dataclasses.dataclass
+class JKProblem:
+    """A batch of body-fixed (J, K) blocks sharing the R, r grids.
+
+    psi is one complex array with the blocks back to back, each
+    n1 x n2 x n_K column-major (R fastest).  Block (J=0, K=0) uses the
+    reference's theta matrices (Wf = (l+1/2) w P_l, Pb = P_l, timeEvol.C:737-760)
+    so that it reproduces the J = 0 propagator exactly; blocks with J > 0 use
+    the normalised associated Legendre FBR so the Coriolis mixing is orthogonal."""
+    base: Problem                  # grids, masses, dt, dumps, energies, surface
+    jmax: int
+    Js: list
+    blocks: list
+    x: dict                        # K -> DVR points
+    omega: dict                    # K -> weights
+    Pb: dict                       # key -> (n, n) Pb[j, k]
+    Wf: dict                       # key -> (n, n) Wf[k, j]
+    pot: dict                      # K -> V (n1*n2*n_K flat)
+    psi: np.ndarray                # complex, all blocks
+    coriolis: bool = True
+
+    @property
+    def grid_points(self):
+        return int(self.psi.size)
+
+    def matrix_key(self, b: JKBlock):
+        return "ref0" if b.J == 0 else b.K
+
+    def mixing(self, J: int):
+        """For each j: (U_j, l-values) of the K coupling at this J."""
+        out = {}
+        for j in range(self.jmax + 1):
+            if self.coriolis:
+                out[j] = coriolis_eigen(J, j)
+            else:
+                nk = min(J, j) + 1
+                U = np.eye(nk)
+                # centrifugal-sudden: diagonal (J-j)^2 -> J(J+1) + j(j+1) - 2K^2 = l(l+1) is not
+                # an integer l in general; carry the diagonal value itself
+                lam = np.array([J * (J + 1) + j * (j + 1) - 2 * K * K for K in range(nk)], dtype=float)
+                out[j] = (U, lam)
+        return out
+
+
+def rot_lambda(J: int, j: int, lvals):
+    """Eigenvalues of (J-j)^2 per eigen slot: l(l+1) for integer l, else the value."""
+    lv = np.asarray(lvals)
+    if lv.dtype.kind in "iu":
+        return (lv * (lv + 1)).astype(float)
+    return lv.astype(float)
+
+
+def make_jk_problem(base: Problem, system: LepsSystem, Js, jmax: int, j0: int = 0, K0: int = 0,
+                    v0: int = 0, R0=None, delta=None, e0_ev=None, coriolis=True) -> JKProblem:
+    """Blocks K = 0..min(J, jmax) for every J in Js on base's R, r grids; the
+    initial packet g(R) phi_v0j0(r) Pt_j0^K0(x) sits in block K0 of every J."""
+    R, r = base.r1, base.r2
+    n1, n2 = R.size, r.size
+    x, om, Pbm, Wfm, pot = {}, {}, {}, {}, {}
+    Kmax = min(max(Js), jmax)
+    for K in range(Kmax + 1):
+        if K == 0:
+            # the Gauss-Legendre grid of the J = 0 propagator, so block (0, 0)
+            # and every K = 0 block see the same potential values
+            xk, ok = gauss_legendre(jmax + 1)
+            Pk = np.sqrt(np.arange(jmax + 1) + 0.5)[:, None] * legendre_p2(jmax, xk)
+        else:
+            xk, ok, Pk = assoc_legendre_dvr(K, jmax)
+        x[K], om[K] = xk, ok
+        Pbm[K] = Pk
+        Wfm[K] = (ok[:, None] * Pk.T)
+        V = system.potential(R[:, None, None], r[None, :, None], xk[None, None, :])
+        pot[K] = np.ascontiguousarray(np.transpose(V, (2, 1, 0))).reshape(-1)
+    if 0 in Js:
+        xg, wg = gauss_legendre(jmax + 1)
+        L = legendre_p2(jmax, xg)
+        Pbm["ref0"] = L
+        Wfm["ref0"] = (wg[:, None] * ((np.arange(jmax + 1) + 0.5)[None, :] * L.T))
+    # initial state
+    delta = delta if delta is not None else 0.3
+    R0 = R0 if R0 is not None else 8.0
+    muR, mur = system.mu_R, system.mu_r
+    k0 = math.sqrt(2 * muR * (e0_ev if e0_ev is not None else 0.2) / H_EV)
+    e_v, phi = diatomic_state(r, r[1] - r[0], mur, system.diatomic_potential(r), j0, v0)
+    g = (1.0 / (np.pi * delta ** 2)) ** 0.25 * np.exp(-(R - R0) ** 2 / (2 * delta * delta) - 1j * k0 * R)
+    blocks = []
+    off = 0
+    for J in Js:
+        for K in range(min(J, jmax) + 1):
+            n = jmax - K + 1
+            blocks.append(JKBlock(J, K, n, off))
+            off += n1 * n2 * n
+    psi = np.zeros(off, dtype=np.complex128)
+    for b in blocks:
+        if b.K != K0 or j0 < K0:
+            continue
+        key = "ref0" if b.J == 0 else b.K
+        # DVR values of Pt_j0^K0: row j0 - K of Pb (normalised) or sqrt(j0+1/2) P_j0 for ref0
+        if key == "ref0":
+            th = math.sqrt(j0 + 0.5) * Pbm["ref0"][j0]
+        else:
+            th = Pbm[K0][j0 - K0]
+        blk = g[:, None, None] * phi[None, :, None] * th[None, None, :]
+        psi[b.offset:b.offset + blk.size] = np.transpose(blk, (2, 1, 0)).reshape(-1)
+    return JKProblem(base=base, jmax=jmax, Js=list(Js), blocks=blocks, x=x, omega=om, Pb=Pbm, Wf=Wfm,
+                     pot=pot, psi=psi, coriolis=coriolis)

@@ -199,3 +199,106 @@ def run(problem: Problem, steps: Optional[int] = None, **opts):
     with Propagator(p, **opts) as prop:
         prop.propagate(p.total_steps)
         return prop.psi(), (prop.crp() if p.calculate_crp else None)
+
+
+def _debug_methods(cls):
+    def debug_stage(self, stage: int):
+        _abi.check(self.lib.qd_debug_stage(self.h, int(stage)))
+
+    def get_fbr(self) -> np.ndarray:
+        out = np.empty(2 * self.n_fbr_points)
+        _abi.check(self.lib.qd_debug_get_fbr(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def set_fbr(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == 2 * self.n_fbr_points
+        _abi.check(self.lib.qd_debug_set_fbr(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def set_psi(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == 2 * self.n_points
+        _abi.check(self.lib.qd_set_psi(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    cls.debug_stage = debug_stage
+    cls.get_fbr = get_fbr
+    cls.set_fbr = set_fbr
+    cls.set_psi = set_psi
+    return cls
+
+
+_debug_methods(Propagator)
+Propagator.n_points = property(lambda self: self.problem.grid_points)
+Propagator.n_fbr_points = property(lambda self: self.problem.n1 * self.problem.n2 * (self.problem.m + 1))
+
+
+def jk_theta_grid(K: int, jmax: int):
+    """The library's theta points and weights of block K (qd_jk_theta_grid)."""
+    lib = _abi.load_library()
+    n = jmax - K + 1
+    x = np.empty(n)
+    w = np.empty(n)
+    _abi.check(lib.qd_jk_theta_grid(int(K), int(jmax), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                    w.ctypes.data_as(C.POINTER(C.c_double))))
+    return x, w
+
+
+@_debug_methods
+class JKPropagator(Propagator):
+    """Handle over a batch of (J, K) blocks (qd_create_jk)."""
+
+    def __init__(self, jk, precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
+                 device=0, profile=0, theta_split=0):
+        self.lib = _abi.load_library()
+        self.jk = jk
+        base = jk.base
+        self.problem = base
+        self._st = _abi.ProblemStruct(base)
+        self._psi_blocks = []
+        blocks = (_abi.qd_jk_block * len(jk.blocks))()
+        n12 = base.n1 * base.n2
+        for i, b in enumerate(jk.blocks):
+            z = np.ascontiguousarray(jk.psi[b.offset:b.offset + n12 * b.n]).view(np.float64).copy()
+            self._psi_blocks.append(z)
+            blocks[i].J = b.J
+            blocks[i].K = b.K
+            blocks[i].pot = jk.pot[b.K].ctypes.data_as(C.POINTER(C.c_double))
+            blocks[i].psi = z.ctypes.data_as(C.POINTER(C.c_double))
+        self._blocks = blocks
+        q = _abi.qd_jk_problem()
+        q.base = self._st.s
+        q.jmax = jk.jmax
+        q.coriolis = 1 if jk.coriolis else 0
+        q.n_blocks = len(jk.blocks)
+        q.blocks = C.cast(blocks, C.POINTER(_abi.qd_jk_block))
+        self._q = q
+        self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile, theta_split)
+        h = C.c_void_p()
+        _abi.check(self.lib.qd_create_jk(C.byref(q), C.byref(self._opt), C.byref(h)))
+        self.h = h
+        self._cb = None
+
+    @property
+    def n_points(self):
+        return int(self.jk.psi.size)
+
+    @property
+    def n_fbr_points(self):
+        return int(self.jk.psi.size)
+
+    def psi(self) -> np.ndarray:
+        out = np.empty(2 * self.n_points)
+        _abi.check(self.lib.qd_get_psi(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def crp_blocks(self) -> np.ndarray:
+        out = np.empty((len(self.jk.blocks), self.problem.energies.size))
+        _abi.check(self.lib.qd_get_crp_blocks(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def crp_by_J(self) -> dict:
+        pb = self.crp_blocks()
+        out = {}
+        for i, b in enumerate(self.jk.blocks):
+            out[b.J] = out.get(b.J, 0) + pb[i]
+        return out
