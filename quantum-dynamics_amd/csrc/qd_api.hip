@@ -248,9 +248,10 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   typedef typename cx_of<T>::type C;
   RPassArgs<T> a;
   a.data = data;
-  a.tw = (const C *)h->twR;
-  a.kin = (const C *)h->kinR;
-  a.dR = (const C *)h->dR;
+  typedef typename tab_of<T>::type TB;
+  a.tw = (const TB *)h->twR;
+  a.kin = (const TB *)h->kinR;
+  a.dR = (const TB *)h->dR;
   a.fcol_lam = h->fcol_lam;
   a.bundle_u0 = h->bundle_u0;
   a.units = h->units;
@@ -272,9 +273,10 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   typedef typename cx_of<T>::type C;
   RLineArgs<T> a;
   a.data = data;
-  a.tw = (const C *)h->twr;
-  a.kin = (const C *)h->kinr;
-  a.dr = (const C *)h->dr;
+  typedef typename tab_of<T>::type TB;
+  a.tw = (const TB *)h->twr;
+  a.kin = (const TB *)h->kinr;
+  a.dr = (const TB *)h->dr;
   a.fcol_j = h->fcol_j;
   a.n1 = h->n1;
   a.n2 = h->n2;
@@ -391,6 +393,7 @@ int apply_prephase(qd_handle *h) {
 template <typename T>
 int step_impl(qd_handle *h) {
   typedef typename cx_of<T>::type C;
+  typedef typename tab_of<T>::type TB;
   C *psi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32);
   C *lpsi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32);
   const bool cuda = h->variant == QD_VARIANT_CUDA;
@@ -417,21 +420,21 @@ int step_impl(qd_handle *h) {
     QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, true, 0, 0));
     QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->dR, (const C *)h->dr,
+    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
                        h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
     QD_RC(mark(h, TK_FFTR1));
     QD_RC(launch_rpass<T>(h, FFT_FWD, lpsi));
     QD_RC(mark(h, TK_FFTR2));
     QD_RC(launch_rline<T>(h, FFT_FWD, lpsi));
     QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_kin2d<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->kinR,
-                       (const C *)h->kinr, h->n1, h->n2, h->nlp);
+    hipLaunchKernelGGL(k_kin2d<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->kinR,
+                       (const TB *)h->kinr, h->n1, h->n2, h->nlp);
     QD_RC(mark(h, TK_FFTR2));
     QD_RC(launch_rline<T>(h, FFT_INV, lpsi));
     QD_RC(mark(h, TK_FFTR1));
     QD_RC(launch_rpass<T>(h, FFT_INV, lpsi));
     QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const C *)h->dR, (const C *)h->dr,
+    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
                        h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
     QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, false, 0, 0));
@@ -519,14 +522,23 @@ int setup_flux(qd_handle *h, const qd_problem *p) {
   return QD_OK;
 }
 
-// Upload a complex table given in FP64 as the handle's precision.
+// Upload a complex table given in FP64: as double2 for the FP64 path, as
+// float4 (hi, lo) pairs for the mixed path (see tab_of in qd_kernels.hpp).
 int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
   if (h->prec == QD_PREC_FP64) {
     double *d = nullptr;
     QD_RC(upload(&d, v.data(), v.size()));
     *dst = d;
   } else {
-    std::vector<float> f(v.begin(), v.end());
+    const size_t n = v.size() / 2;
+    std::vector<float> f(4 * n);
+    for (size_t i = 0; i < n; i++) {
+      const float hx = (float)v[2 * i], hy = (float)v[2 * i + 1];
+      f[4 * i] = hx;
+      f[4 * i + 1] = hy;
+      f[4 * i + 2] = (float)(v[2 * i] - (double)hx);
+      f[4 * i + 3] = (float)(v[2 * i + 1] - (double)hy);
+    }
     float *d = nullptr;
     QD_RC(upload(&d, f.data(), f.size()));
     *dst = d;

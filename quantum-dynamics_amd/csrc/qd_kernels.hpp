@@ -70,6 +70,33 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
   return r;
 }
 
+// Fixed operator tables (FFT twiddles, kinetic and rotational phase factors).
+// FP64 tables hold the value; FP32 tables hold hi + lo, hi = fl32(w) and
+// lo = fl32(w - hi), applied with fused multiply-adds so that the FP32 product
+// is the rounding of a*w rather than of a*fl32(w).  The same few twiddles and
+// phase factors multiply every line of every step, so their FP32 rounding
+// (|fl32(w)|^2 - 1 ~ 1e-8) would otherwise shrink or grow the norm coherently.
+template <typename T> struct tab_of;
+template <> struct tab_of<double> { typedef double2 type; };
+template <> struct tab_of<float> { typedef float4 type; };
+
+__device__ __forceinline__ double2 tmul(double2 a, double2 w, bool conj_w) {
+  if (conj_w) w.y = -w.y;
+  return cmul(a, w);
+}
+__device__ __forceinline__ float2 tmul(float2 a, float4 w, bool conj_w) {
+  const float wy = conj_w ? -w.y : w.y, ly = conj_w ? -w.w : w.w;
+  const float tx = a.x * w.z - a.y * ly;
+  const float ty = a.x * ly + a.y * w.z;
+  float2 r;
+  r.x = fmaf(a.x, w.x, fmaf(-a.y, wy, tx));
+  r.y = fmaf(a.x, wy, fmaf(a.y, w.x, ty));
+  return r;
+}
+// x * s for a real constant given as hi + lo (FP32) or a double
+__device__ __forceinline__ float csmul(float x, float hi, float lo) { return fmaf(x, hi, x * lo); }
+__device__ __forceinline__ double csmul(double x, double hi, double) { return x * hi; }
+
 // ---------------------------------------------------------------------------
 // Batch description
 // ---------------------------------------------------------------------------
@@ -161,8 +188,8 @@ __global__ void k_dump(typename cx_of<T>::type *psi, const double *d1, const dou
 
 // lpsi *= D_R/2[lam(c)][i] D_r/2[jr(c)][j]  (evolution_with_rotational_dt_2, timeEvol.C:561-582)
 template <typename T>
-__global__ void k_rot(typename cx_of<T>::type *lpsi, const typename cx_of<T>::type *dR,
-                      const typename cx_of<T>::type *dr, const int *fcol_lam, const int *fcol_j, int n1, int n2,
+__global__ void k_rot(typename cx_of<T>::type *lpsi, const typename tab_of<T>::type *dR,
+                      const typename tab_of<T>::type *dr, const int *fcol_lam, const int *fcol_j, int n1, int n2,
                       size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
@@ -170,19 +197,19 @@ __global__ void k_rot(typename cx_of<T>::type *lpsi, const typename cx_of<T>::ty
   const size_t jc = x / n1;
   const int j = (int)(jc % n2);
   const int c = (int)(jc / n2);
-  lpsi[x] = cmul(lpsi[x], cmul(dR[(size_t)fcol_lam[c] * n1 + i], dr[(size_t)fcol_j[c] * n2 + j]));
+  lpsi[x] = tmul(tmul(lpsi[x], dR[(size_t)fcol_lam[c] * n1 + i], false), dr[(size_t)fcol_j[c] * n2 + j], false);
 }
 
 // lpsi *= kinR[i] kinr[j]  (evolution_with_kinetic_dt, timeEvol.C:584-606; the
 // tables carry 1/n1 and 1/n2)
 template <typename T>
-__global__ void k_kin2d(typename cx_of<T>::type *lpsi, const typename cx_of<T>::type *kinR,
-                        const typename cx_of<T>::type *kinr, int n1, int n2, size_t n) {
+__global__ void k_kin2d(typename cx_of<T>::type *lpsi, const typename tab_of<T>::type *kinR,
+                        const typename tab_of<T>::type *kinr, int n1, int n2, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const int i = (int)(x % n1);
   const int j = (int)((x / n1) % n2);
-  lpsi[x] = cmul(lpsi[x], cmul(kinR[i], kinr[j]));
+  lpsi[x] = tmul(tmul(lpsi[x], kinR[i], false), kinr[j], false);
 }
 
 __global__ void k_f64_to_f32(const double2 *a, float2 *b, size_t n) {
@@ -232,26 +259,40 @@ __device__ __forceinline__ void dft_small(C *v, int r, T sign) {
     v[1] = cadd(t1, wd);
     v[3] = csub(t1, wd);
   } else if (r == 3) {
-    const T s3 = sign * (T)0.86602540378443864676372317075294;
+    // sin(2 pi/3) as hi + lo in FP32
+    const T s3h = (T)0.86602540378443864676372317075294;
+    const T s3l = (T)(0.86602540378443864676372317075294 - (double)s3h);
     const C t = cadd(v[1], v[2]);
     const C d = csub(v[1], v[2]);
     const C a0 = v[0];
     v[0] = cadd(a0, t);
     const C mm = csub(a0, cscale(t, (T)0.5));
-    const C jd = cmul_i(d, s3);
+    C ds;
+    ds.x = csmul(d.x, s3h, s3l);
+    ds.y = csmul(d.y, s3h, s3l);
+    const C jd = cmul_i(ds, sign);
     v[1] = cadd(mm, jd);
     v[2] = csub(mm, jd);
   } else if (r == 5) {
-    const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
-    const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
+    const double C1 = 0.30901699437494742410229341718282, C2 = -0.80901699437494742410229341718282;
+    const double S1 = 0.95105651629515357211643933337938, S2 = 0.58778525229247312916870595463907;
+    const T c1 = (T)C1, c2 = (T)C2, s1 = (T)S1, s2 = (T)S2;
+    const T c1l = (T)(C1 - (double)c1), c2l = (T)(C2 - (double)c2), s1l = (T)(S1 - (double)s1),
+            s2l = (T)(S2 - (double)s2);
+    auto sc = [](C z, T h, T l) {
+      C r;
+      r.x = csmul(z.x, h, l);
+      r.y = csmul(z.y, h, l);
+      return r;
+    };
     const C a0 = v[0];
     const C b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
     const C d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
     v[0] = cadd(a0, cadd(b1, b2));
-    const C m1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
-    const C m2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
-    const C n1 = cmul_i(cadd(cscale(d1, s1), cscale(d2, s2)), sign);
-    const C n2 = cmul_i(csub(cscale(d1, s2), cscale(d2, s1)), sign);
+    const C m1 = cadd(a0, cadd(sc(b1, c1, c1l), sc(b2, c2, c2l)));
+    const C m2 = cadd(a0, cadd(sc(b1, c2, c2l), sc(b2, c1, c1l)));
+    const C n1 = cmul_i(cadd(sc(d1, s1, s1l), sc(d2, s2, s2l)), sign);
+    const C n2 = cmul_i(csub(sc(d1, s2, s2l), sc(d2, s1, s1l)), sign);
     v[1] = cadd(m1, n1);
     v[4] = csub(m1, n1);
     v[2] = cadd(m2, n2);
@@ -266,7 +307,7 @@ __device__ __forceinline__ void dft_small(C *v, int r, T sign) {
 template <typename T, typename LIDX, typename MUL>
 __device__ __forceinline__ void stockham_pass(typename cx_of<T>::type *lds, LIDX lidx, int nlines, int slot,
                                               int nslots, int bt, int tpl, int n, int r, int Ns, bool fwd,
-                                              const typename cx_of<T>::type *tw, MUL post) {
+                                              const typename tab_of<T>::type *tw, MUL post) {
   typedef typename cx_of<T>::type C;
   const T sign = fwd ? (T)-1 : (T)1;
   const int nb = n / r;
@@ -295,9 +336,7 @@ __device__ __forceinline__ void stockham_pass(typename cx_of<T>::type *lds, LIDX
 #pragma unroll
         for (int q = 1; q < 5; q++) {
           if (q >= r) break;
-          C w = tw[q * estep];
-          if (!fwd) w.y = -w.y;
-          v[bb][q] = cmul(v[bb][q], w);
+          v[bb][q] = tmul(v[bb][q], tw[q * estep], !fwd);
         }
       }
       dft_small<C, T>(v[bb], r, sign);
@@ -333,9 +372,9 @@ struct RUnit {
 template <typename T>
 struct RPassArgs {
   typename cx_of<T>::type *data;
-  const typename cx_of<T>::type *tw;   // exp(-2 pi i e / n1)
-  const typename cx_of<T>::type *kin;  // exp(-i T1 dt) / n1
-  const typename cx_of<T>::type *dR;   // D_R/2 table [lam][n1]
+  const typename tab_of<T>::type *tw;   // exp(-2 pi i e / n1)
+  const typename tab_of<T>::type *kin;  // exp(-i T1 dt) / n1
+  const typename tab_of<T>::type *dR;   // D_R/2 table [lam][n1]
   const int *fcol_lam;                 // lam index per FBR column (in the l basis)
   const int *bundle_u0;                // [nbundle + 1] range of units per bundle
   const RUnit *units;
@@ -406,12 +445,12 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
 #pragma unroll
           for (int K = 0; K < RP_GMAX; K++)
             if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
-          if (phase) acc = cmul(acc, A.dR[(size_t)s_lam[L0 + m] * n + p]);
+          if (phase) acc = tmul(acc, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
           lds[(L0 + m) * n + p] = acc;
         }
       } else if (phase) {
         for (int m = 0; m < U.g; m++)
-          lds[(L0 + m) * n + p] = cmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p]);
+          lds[(L0 + m) * n + p] = tmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p], false);
       }
     }
     __syncthreads();
@@ -429,7 +468,7 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
       if (pi == np - 1 && (A.flags & FFT_KIN)) {
-        auto kin = [&](int, int p, C x) { return cmul(x, A.kin[p]); };
+        auto kin = [&](int, int p, C x) { return tmul(x, A.kin[p], false); };
         stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, kin);
       } else {
         stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, keep);
@@ -457,7 +496,7 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
         for (int m = 0; m < RP_GMAX; m++)
           if (m < U.g) {
             C x = lds[(L0 + m) * n + p];
-            if (phase) x = cmul(x, A.dR[(size_t)s_lam[L0 + m] * n + p]);
+            if (phase) x = tmul(x, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
             in[m] = x;
           }
 #pragma unroll 1
@@ -472,7 +511,7 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
         }
       } else if (phase) {
         for (int m = 0; m < U.g; m++)
-          lds[(L0 + m) * n + p] = cmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p]);
+          lds[(L0 + m) * n + p] = tmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p], false);
       }
     }
     __syncthreads();
@@ -490,9 +529,9 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
 template <typename T>
 struct RLineArgs {
   typename cx_of<T>::type *data;
-  const typename cx_of<T>::type *tw;   // exp(-2 pi i e / n2)
-  const typename cx_of<T>::type *kin;  // exp(-i T2 dt) / n2
-  const typename cx_of<T>::type *dr;   // D_r/2 table [j][n2]
+  const typename tab_of<T>::type *tw;   // exp(-2 pi i e / n2)
+  const typename tab_of<T>::type *kin;  // exp(-i T2 dt) / n2
+  const typename tab_of<T>::type *dr;   // D_r/2 table [j][n2]
   const int *fcol_j;
   int n1, n2;
   int lpb;
@@ -528,7 +567,7 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
     x.y = 0;
     if (active) {
       x = A.data[base + (size_t)p * stride];
-      if (A.flags & FFT_PHASE) x = cmul(x, A.dr[(size_t)jphase * n + p]);
+      if (A.flags & FFT_PHASE) x = tmul(x, A.dr[(size_t)jphase * n + p], false);
     }
     lds[p * lpb + line_local] = x;
   }
@@ -541,7 +580,7 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
       if (pi == np - 1 && (A.flags & FFT_KIN)) {
-        auto kin = [&](int, int p, C x) { return cmul(x, A.kin[p]); };
+        auto kin = [&](int, int p, C x) { return tmul(x, A.kin[p], false); };
         stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, kin);
       } else {
         stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, keep);
@@ -560,7 +599,7 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
   for (int p = bt; p < n; p += tpl) {
     if (active) {
       C x = lds[p * lpb + line_local];
-      if (A.flags & FFT_PHASE) x = cmul(x, A.dr[(size_t)jphase * n + p]);
+      if (A.flags & FFT_PHASE) x = tmul(x, A.dr[(size_t)jphase * n + p], false);
       A.data[base + (size_t)p * stride] = x;
     }
   }
