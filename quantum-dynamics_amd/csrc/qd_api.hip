@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/qd.h"
+#include "qd_fft.hpp"
 #include "qd_kernels.hpp"
 #include "qd_tables.hpp"
 
@@ -184,7 +185,8 @@ struct qd_handle {
   RUnit *units = nullptr;
   int *unit_cols = nullptr;
   void *umat = nullptr;
-  int nbundle = 0, rp_tpl = 16;
+  int nbundle = 0, rp_tpl = 16, rp_ml = 8;
+  bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
   FftPlanArg planR{}, planr{};
   int rl_lpb = 16, rl_blocks = 0;
@@ -242,6 +244,57 @@ int mark(qd_handle *h, int cls) {
   return QD_OK;
 }
 
+// ---- compile-time FFT dispatch ----
+#define QD_CT_LENGTHS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+bool ct_length(int n) {
+  switch (n) {
+#define QD_CASE(N) case N:
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <typename T, int N, int ML>
+const void *rpass_ct_fn(bool mix) {
+  if constexpr (256 / ML * FFT_BPT >= ct_maxnb(N))
+    return mix ? (const void *)k_rpass_ct<T, N, ML, true> : (const void *)k_rpass_ct<T, N, ML, false>;
+  else return nullptr;
+}
+template <typename T, int N, int LPB>
+const void *rline_ct_fn() {
+  if constexpr (256 / LPB * FFT_BPT >= ct_maxnb(N)) return (const void *)k_rline_ct<T, N, LPB>;
+  else return nullptr;
+}
+// kernel for (length, lines per group); nullptr -> generic kernel
+template <typename T>
+const void *rpass_kernel(int n, int ml, bool mix) {
+  switch (n) {
+#define QD_CASE(N) \
+  case N:          \
+    return ml == 8 ? rpass_ct_fn<T, N, 8>(mix) : ml == 32 ? rpass_ct_fn<T, N, 32>(mix) : nullptr;
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+template <typename T>
+const void *rline_kernel(int n, int lpb) {
+  switch (n) {
+#define QD_CASE(N) \
+  case N:          \
+    return lpb == 16 ? rline_ct_fn<T, N, 16>() : lpb == 8 ? rline_ct_fn<T, N, 8>() : nullptr;
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+
 // ---- launches ----
 template <typename T>
 int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
@@ -263,7 +316,13 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.flags = flags;
   if (h->has_mix && (flags & FFT_PHASE)) a.flags |= FFT_MIX;
   a.plan = h->planR;
-  hipLaunchKernelGGL((k_rpass<T>), dim3(h->nbundle), dim3(RP_THREADS), h->rp_lds, h->stream, a);
+  const void *fn = h->rp_ct ? rpass_kernel<T>(h->n1, h->rp_ml, h->has_mix) : nullptr;
+  if (fn) {
+    void *args[] = {&a};
+    QD_HIP(hipLaunchKernel(fn, dim3(h->nbundle), dim3(256), args, h->rp_lds, h->stream));
+  } else {
+    hipLaunchKernelGGL((k_rpass<T>), dim3(h->nbundle), dim3(RP_THREADS), h->rp_lds, h->stream, a);
+  }
   QD_HIP(hipGetLastError());
   return QD_OK;
 }
@@ -284,7 +343,13 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.nlines = h->n1 * h->nfcol;
   a.flags = flags;
   a.plan = h->planr;
-  hipLaunchKernelGGL((k_rline<T>), dim3(h->rl_blocks), dim3(256), h->rl_lds, h->stream, a);
+  const void *fn = h->rl_ct ? rline_kernel<T>(h->n2, h->rl_lpb) : nullptr;
+  if (fn) {
+    void *args[] = {&a};
+    QD_HIP(hipLaunchKernel(fn, dim3(h->rl_blocks), dim3(256), args, h->rl_lds, h->stream));
+  } else {
+    hipLaunchKernelGGL((k_rline<T>), dim3(h->rl_blocks), dim3(256), h->rl_lds, h->stream, a);
+  }
   QD_HIP(hipGetLastError());
   return QD_OK;
 }
@@ -612,15 +677,32 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   if (gmax > RP_GMAX) return fail(QD_ERR_ARG, "mixing groups larger than 32 (J > 31) are not supported");
   std::vector<int> bu0, ucols;
   std::vector<RUnit> units;
+  const size_t tabsz = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float4);
   {
+    // R pass: bundles of whole units; about 2048 elements per work group, at
+    // least one mixing group, one line per thread slot.  Lengths with a
+    // compile-time kernel use 8 or 32 lines per group.
     const int mb = max_butterflies(h->planR);
-    const int tpl = std::max(1, pow2ceil((mb + FFT_BPT - 1) / FFT_BPT));
-    if (tpl > RP_THREADS) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n1) + " too large");
-    int ml = std::min(RP_THREADS / tpl, 64);
-    const int ml_lds = (int)(160 * 1024 / ((size_t)n1 * elem));
-    const int ml_64k = (int)(65536 / ((size_t)n1 * elem));
-    if (std::min(ml, ml_lds) < gmax) return fail(QD_ERR_ARG, "R grid too large for the K-mixing groups");
-    ml = std::min(ml, ml_64k >= gmax ? ml_64k : ml_lds);
+    int ml = 1;
+    while (ml * 2 <= 64 && (size_t)ml * 2 * n1 <= 2048) ml *= 2;
+    ml = std::max(ml, pow2ceil(gmax));
+    bool use_ct = false;
+    if (ct_length(n1)) {
+      const int mlc = (gmax > 8 || n1 <= 64) ? 32 : 8;
+      const bool fits = (h->prec == QD_PREC_FP64 ? rpass_kernel<double>(n1, mlc, h->has_mix) : rpass_kernel<float>(n1, mlc, h->has_mix)) != nullptr;
+      if (fits && mlc >= gmax) {
+        ml = mlc;
+        use_ct = true;
+      }
+    }
+    if (ml > 64) return fail(QD_ERR_ARG, "mixing groups too large for the R pass");
+    const int tpl = RP_THREADS / ml;
+    if (tpl * FFT_BPT < mb) return fail(QD_ERR_ARG, "R grid too large for the R pass with these mixing groups");
+    const size_t pad_stride = (size_t)n1 + n1 / 32 + 1;
+    const size_t lds = 2 * (size_t)n1 * tabsz + (size_t)ml * (use_ct ? pad_stride : (size_t)n1) * elem;
+    if (lds > 160 * 1024) return fail(QD_ERR_ARG, "R grid too large for the R pass (LDS)");
+    h->rp_ct = use_ct;
+    h->rp_ml = ml;
     h->rp_tpl = tpl;
     int lines = 0, nu = 0;
     bu0.push_back(0);
@@ -646,17 +728,33 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     }
     bu0.push_back((int)units.size());
     h->nbundle = (int)bu0.size() - 1;
-    h->rp_lds = (size_t)ml * n1 * elem;
+    h->rp_lds = lds;
   }
   {
+    // r pass: lpb consecutive i of one column (lpb divides n1)
     const int mb = max_butterflies(h->planr);
     int lpb = 16;
-    while (lpb > 1 && ((256 / lpb) * FFT_BPT < mb || (size_t)lpb * n2 * elem > 65536)) lpb /= 2;
-    if ((256 / lpb) * FFT_BPT < mb || (size_t)lpb * n2 * elem > 65536)
-      return fail(QD_ERR_ARG, "FFT length " + std::to_string(n2) + " too large");
+    auto ok = [&](int l) {
+      return n1 % l == 0 && (256 / l) * FFT_BPT >= mb && 3 * (size_t)n2 * tabsz + (size_t)l * n2 * elem <= 65536;
+    };
+    while (lpb > 1 && !ok(lpb)) lpb /= 2;
+    if (!ok(lpb)) return fail(QD_ERR_ARG, "FFT length " + std::to_string(n2) + " too large");
     h->rl_lpb = lpb;
-    h->rl_blocks = (int)(((size_t)n1 * fcol + lpb - 1) / lpb);
-    h->rl_lds = (size_t)lpb * n2 * elem;
+    h->rl_ct = false;
+    if (ct_length(n2)) {
+      for (int l : {16, 8}) {
+        const size_t ldsc = 3 * (size_t)n2 * tabsz + (size_t)n2 * (l + 1) * elem;
+        const bool fits = (h->prec == QD_PREC_FP64 ? rline_kernel<double>(n2, l) : rline_kernel<float>(n2, l)) != nullptr;
+        if (fits && n1 % l == 0 && ldsc <= 160 * 1024) {
+          h->rl_lpb = l;
+          h->rl_ct = true;
+          break;
+        }
+      }
+    }
+    h->rl_blocks = (int)(((size_t)n1 * fcol + h->rl_lpb - 1) / h->rl_lpb);
+    h->rl_lds = h->rl_ct ? 3 * (size_t)n2 * tabsz + (size_t)n2 * (h->rl_lpb + 1) * elem
+                         : 3 * (size_t)n2 * tabsz + (size_t)h->rl_lpb * n2 * elem;
   }
   // theta matrices: padded shapes shared by all sets
   h->ncb = (h->prec == QD_PREC_MIXED && maxn <= 64) ? 4 : 8;
@@ -676,12 +774,13 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     for (auto &e : h->marks) QD_HIP(hipEventCreate(&e));
   }
   if (h->rp_lds > 65536) {
-    if (h->prec == QD_PREC_FP64)
-      QD_HIP(hipFuncSetAttribute((const void *)k_rpass<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)h->rp_lds));
-    else
-      QD_HIP(hipFuncSetAttribute((const void *)k_rpass<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)h->rp_lds));
+    const void *fn = h->rp_ct ? (h->prec == QD_PREC_FP64 ? rpass_kernel<double>(n1, h->rp_ml, h->has_mix) : rpass_kernel<float>(n1, h->rp_ml, h->has_mix))
+                              : (h->prec == QD_PREC_FP64 ? (const void *)k_rpass<double> : (const void *)k_rpass<float>);
+    QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->rp_lds));
+  }
+  if (h->rl_lds > 65536) {
+    const void *fn = h->prec == QD_PREC_FP64 ? rline_kernel<double>(n2, h->rl_lpb) : rline_kernel<float>(n2, h->rl_lpb);
+    QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->rl_lds));
   }
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));

@@ -228,7 +228,7 @@ __global__ void k_f32_to_f64(const float2 *a, double2 *b, size_t n) {
 // w^(q (b mod Ns) n/(Ns r)), and writes y[(b/Ns) Ns r + (b mod Ns) + q Ns]
 // (autosort Stockham; any order of the radices gives the DFT).
 constexpr int FFT_MAX_PASS = 12;
-constexpr int FFT_BPT = 4;  // butterflies per thread per pass (max)
+constexpr int FFT_BPT = 8;  // butterflies per thread per pass (max)
 
 struct FftPlanArg {
   int n;
@@ -358,8 +358,10 @@ __device__ __forceinline__ void stockham_pass(typename cx_of<T>::type *lds, LIDX
 // A work group takes one bundle of lines.  Lines come in units: a unit is the
 // g FBR lines (J, K = 0..g-1, j) at one r index, mixed by the g x g matrix U
 // of (J, j): c_m = sum_K U[K][m] c_K before the transform and c_K = sum_m
-// U[K][m] c_m after it.  For J = 0 (and j = 0) units are single lines.
-constexpr int RP_THREADS = 512;
+// U[K][m] c_m after it.  For J = 0 (and j = 0) units are single lines and the
+// phase factors ride on the global load and store.  Twiddles and the kinetic
+// factor sit in LDS; each line has `tpl` threads for its whole transform.
+constexpr int RP_THREADS = 256;
 constexpr int RP_GMAX = 32;  // largest unit (mixing group)
 
 struct RUnit {
@@ -389,16 +391,21 @@ struct RPassArgs {
 template <typename T>
 __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
   typedef typename cx_of<T>::type C;
+  typedef typename tab_of<T>::type TB;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C *lds = reinterpret_cast<C *>(smem_raw);
-  __shared__ long long s_off[64];   // global offset of each line of the bundle
-  __shared__ int s_lam[64];
-  __shared__ int s_unit_line0[65];
   const int n = A.n1;
+  TB *s_tw = reinterpret_cast<TB *>(smem_raw);
+  TB *s_kin = s_tw + n;
+  C *lds = reinterpret_cast<C *>(s_kin + n);
+  __shared__ long long s_off[64];  // global offset of each line of the bundle
+  __shared__ int s_lam[64];
+  __shared__ int s_mixed[64];      // line belongs to a unit that is mixed
+  __shared__ int s_unit_line0[65];
   const int t = threadIdx.x;
   const int ub = A.bundle_u0[blockIdx.x], ue = A.bundle_u0[blockIdx.x + 1];
   const int nu = ue - ub;
-  // line table of the bundle (nu <= 64 units, lines <= 64)
+  const bool mix = A.flags & FFT_MIX;
+  const bool phase = A.flags & FFT_PHASE;
   if (t == 0) {
     int L = 0;
     for (int u = 0; u < nu; u++) {
@@ -406,6 +413,10 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
       L += A.units[ub + u].g;
     }
     s_unit_line0[nu] = L;
+  }
+  for (int e = t; e < n; e += RP_THREADS) {
+    s_tw[e] = A.tw[e];
+    s_kin[e] = A.kin[e];
   }
   __syncthreads();
   const int nl = s_unit_line0[nu];
@@ -416,41 +427,38 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
     const int col = A.unit_cols[U.col0 + (t - s_unit_line0[u])];
     s_off[t] = ((long long)col * A.n2 + U.r) * n;
     s_lam[t] = A.fcol_lam[col];
+    s_mixed[t] = (mix && U.u0 >= 0) ? 1 : 0;
   }
   __syncthreads();
-  // load
+  // load; D_R/2 of unmixed lines rides on the load
   for (int e = t; e < nl * n; e += RP_THREADS) {
-    const int L = e / n, p = e - (e / n) * n;
-    lds[L * n + p] = A.data[s_off[L] + p];
+    const int L = e / n, p = e - L * n;
+    C x = A.data[s_off[L] + p];
+    if (phase && !s_mixed[L]) x = tmul(x, A.dR[(size_t)s_lam[L] * n + p], false);
+    lds[L * n + p] = x;
   }
   __syncthreads();
-  const bool mix = A.flags & FFT_MIX;
-  const bool phase = A.flags & FFT_PHASE;
-  // K -> l mixing and D_R/2 (one thread per unit position)
-  if (mix || phase) {
+  // K -> l mixing and D_R/2 of mixed units (one thread per unit position)
+  if (mix) {
     for (int e = t; e < nu * n; e += RP_THREADS) {
-      const int u = e / n, p = e - (e / n) * n;
+      const int u = e / n, p = e - u * n;
       const RUnit U = A.units[ub + u];
+      if (U.u0 < 0) continue;
       const int L0 = s_unit_line0[u];
-      if (mix && U.u0 >= 0) {
-        C in[RP_GMAX];
+      C in[RP_GMAX];
+#pragma unroll
+      for (int K = 0; K < RP_GMAX; K++)
+        if (K < U.g) in[K] = lds[(L0 + K) * n + p];
+#pragma unroll 1
+      for (int m = 0; m < U.g; m++) {
+        C acc;
+        acc.x = 0;
+        acc.y = 0;
 #pragma unroll
         for (int K = 0; K < RP_GMAX; K++)
-          if (K < U.g) in[K] = lds[(L0 + K) * n + p];
-#pragma unroll 1
-        for (int m = 0; m < U.g; m++) {
-          C acc;
-          acc.x = 0;
-          acc.y = 0;
-#pragma unroll
-          for (int K = 0; K < RP_GMAX; K++)
-            if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
-          if (phase) acc = tmul(acc, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
-          lds[(L0 + m) * n + p] = acc;
-        }
-      } else if (phase) {
-        for (int m = 0; m < U.g; m++)
-          lds[(L0 + m) * n + p] = tmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p], false);
+          if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
+        if (phase) acc = tmul(acc, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
+        lds[(L0 + m) * n + p] = acc;
       }
     }
     __syncthreads();
@@ -468,10 +476,10 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
       if (pi == np - 1 && (A.flags & FFT_KIN)) {
-        auto kin = [&](int, int p, C x) { return tmul(x, A.kin[p], false); };
-        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, kin);
+        auto kin = [&](int, int p, C x) { return tmul(x, s_kin[p], false); };
+        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, s_tw, kin);
       } else {
-        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, A.tw, keep);
+        stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, true, s_tw, keep);
       }
       Ns *= r;
     }
@@ -480,51 +488,52 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
     int Ns = 1;
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
-      stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, false, A.tw, keep);
+      stockham_pass<T>(lds, lidx, nl, slot, nslots, bt, tpl, n, r, Ns, false, s_tw, keep);
       Ns *= r;
     }
   }
-  // D_R/2 and l -> K mixing
-  if (mix || phase) {
+  // D_R/2 and l -> K mixing of mixed units
+  if (mix) {
     for (int e = t; e < nu * n; e += RP_THREADS) {
-      const int u = e / n, p = e - (e / n) * n;
+      const int u = e / n, p = e - u * n;
       const RUnit U = A.units[ub + u];
+      if (U.u0 < 0) continue;
       const int L0 = s_unit_line0[u];
-      if (mix && U.u0 >= 0) {
-        C in[RP_GMAX];
+      C in[RP_GMAX];
+#pragma unroll
+      for (int m = 0; m < RP_GMAX; m++)
+        if (m < U.g) {
+          C x = lds[(L0 + m) * n + p];
+          if (phase) x = tmul(x, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
+          in[m] = x;
+        }
+#pragma unroll 1
+      for (int K = 0; K < U.g; K++) {
+        C acc;
+        acc.x = 0;
+        acc.y = 0;
 #pragma unroll
         for (int m = 0; m < RP_GMAX; m++)
-          if (m < U.g) {
-            C x = lds[(L0 + m) * n + p];
-            if (phase) x = tmul(x, A.dR[(size_t)s_lam[L0 + m] * n + p], false);
-            in[m] = x;
-          }
-#pragma unroll 1
-        for (int K = 0; K < U.g; K++) {
-          C acc;
-          acc.x = 0;
-          acc.y = 0;
-#pragma unroll
-          for (int m = 0; m < RP_GMAX; m++)
-            if (m < U.g) acc = cadd(acc, cscale(in[m], A.umat[U.u0 + K * U.g + m]));
-          lds[(L0 + K) * n + p] = acc;
-        }
-      } else if (phase) {
-        for (int m = 0; m < U.g; m++)
-          lds[(L0 + m) * n + p] = tmul(lds[(L0 + m) * n + p], A.dR[(size_t)s_lam[L0 + m] * n + p], false);
+          if (m < U.g) acc = cadd(acc, cscale(in[m], A.umat[U.u0 + K * U.g + m]));
+        lds[(L0 + K) * n + p] = acc;
       }
     }
     __syncthreads();
   }
+  // store; D_R/2 of unmixed lines rides on the store
   for (int e = t; e < nl * n; e += RP_THREADS) {
-    const int L = e / n, p = e - (e / n) * n;
-    A.data[s_off[L] + p] = lds[L * n + p];
+    const int L = e / n, p = e - L * n;
+    C x = lds[L * n + p];
+    if (phase && !s_mixed[L]) x = tmul(x, A.dR[(size_t)s_lam[L] * n + p], false);
+    A.data[s_off[L] + p] = x;
   }
 }
 
 // ---------------------------------------------------------------------------
 // r pass: lines along r (stride n1); a work group takes LPB consecutive i of
-// one FBR column so global accesses are coalesced across the group
+// one FBR column so global accesses are coalesced across the group.  Twiddles,
+// the kinetic factor and (when the group lies in one column) its D_r/2 row
+// are staged in LDS.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct RLineArgs {
@@ -543,9 +552,13 @@ struct RLineArgs {
 template <typename T>
 __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
   typedef typename cx_of<T>::type C;
+  typedef typename tab_of<T>::type TB;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C *lds = reinterpret_cast<C *>(smem_raw);
   const int n = A.n2;
+  TB *s_tw = reinterpret_cast<TB *>(smem_raw);
+  TB *s_kin = s_tw + n;
+  TB *s_dr = s_kin + n;
+  C *lds = reinterpret_cast<C *>(s_dr + n);
   const int lpb = A.lpb;
   const int tpl = 256 / lpb;
   const int t = threadIdx.x;
@@ -558,32 +571,39 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
   const int c = lc / A.n1;
   const size_t base = (size_t)c * A.n1 * A.n2 + i;
   const size_t stride = A.n1;
-  const int jphase = A.fcol_j[c];
+  // one column per group when lpb divides n1 (the host arranges it)
+  const int c0 = (blockIdx.x * lpb) / A.n1;
+  const int j0 = A.fcol_j[min(c0, A.nlines / A.n1 - 1)];
+  const bool phase = A.flags & FFT_PHASE;
+  for (int e = t; e < n; e += 256) {
+    s_tw[e] = A.tw[e];
+    s_kin[e] = A.kin[e];
+    if (phase) s_dr[e] = A.dr[(size_t)j0 * n + e];
+  }
+  __syncthreads();
   auto lidx = [lpb, line_local](int, int p) { return p * lpb + line_local; };
-  // load (the element order p*lpb + line_local keeps the group coalesced)
   for (int p = bt; p < n; p += tpl) {
     C x;
     x.x = 0;
     x.y = 0;
     if (active) {
       x = A.data[base + (size_t)p * stride];
-      if (A.flags & FFT_PHASE) x = tmul(x, A.dr[(size_t)jphase * n + p], false);
+      if (phase) x = tmul(x, s_dr[p], false);
     }
     lds[p * lpb + line_local] = x;
   }
   __syncthreads();
   auto keep = [](int, int, C x) { return x; };
   const int np = A.plan.npass;
-  // each "slot" is a line; one line per thread group of tpl
   if (A.flags & FFT_FWD) {
     int Ns = 1;
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
       if (pi == np - 1 && (A.flags & FFT_KIN)) {
-        auto kin = [&](int, int p, C x) { return tmul(x, A.kin[p], false); };
-        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, kin);
+        auto kin = [&](int, int p, C x) { return tmul(x, s_kin[p], false); };
+        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, s_tw, kin);
       } else {
-        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, A.tw, keep);
+        stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, true, s_tw, keep);
       }
       Ns *= r;
     }
@@ -592,14 +612,14 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
     int Ns = 1;
     for (int pi = 0; pi < np; pi++) {
       const int r = A.plan.radix[pi];
-      stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, false, A.tw, keep);
+      stockham_pass<T>(lds, lidx, lpb, line_local, lpb, bt, tpl, n, r, Ns, false, s_tw, keep);
       Ns *= r;
     }
   }
   for (int p = bt; p < n; p += tpl) {
     if (active) {
       C x = lds[p * lpb + line_local];
-      if (A.flags & FFT_PHASE) x = tmul(x, A.dr[(size_t)jphase * n + p], false);
+      if (phase) x = tmul(x, s_dr[p], false);
       A.data[base + (size_t)p * stride] = x;
     }
   }
