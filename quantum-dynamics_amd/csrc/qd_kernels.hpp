@@ -681,13 +681,16 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
 #pragma unroll
     for (int b = 0; b < 8; b++) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = 0; k0 < K; k0 += TG_KC64) {
+  // The next K chunk is fetched into registers (A with the phase prologue
+  // applied) while the current one is on the matrix cores.
+  const int cr = t & 63;
+  const int r = row0 + cr;
+  double2 pa[4];
+  double pb[8];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      const int cr = t & 63;
-      const int kk = (t >> 6) + 4 * s;
-      const int r = row0 + cr;
-      const int k = k0 + kk;
+      const int k = k0 + (t >> 6) + 4 * s;
       double2 v = make_double2(0.0, 0.0);
       if (r < R && k < K) {
         v = A[(size_t)r + (size_t)R * k];
@@ -702,19 +705,24 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
           }
         }
       }
-      *reinterpret_cast<double2 *>(&As[kk][2 * cr]) = v;
+      pa[s] = v;
     }
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int col = t & 127;
-      const int kk = (t >> 7) + 2 * s;
-      Bs[kk][col] = Bp[(size_t)(k0 + kk) * P.Npad + col0 + col];
-    }
+    for (int s = 0; s < 8; s++) pb[s] = Bp[(size_t)(k0 + (t >> 7) + 2 * s) * P.Npad + col0 + (t & 127)];
+  };
+  // v_mfma_f64_16x16x4f64 returns D[(lane/16) + 4v][lane%16] in register v.
+  // Feeding MFMA row i from tile row 4(i%4) + i/4 makes register v of lane l
+  // hold tile row 4(l/16) + v, so each lane ends with two whole complex rows.
+  const int arow = 4 * (lane & 3) + ((lane & 15) >> 2);
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += TG_KC64) {
+    __syncthreads();  // the previous chunk's operands have been read
+#pragma unroll
+    for (int s = 0; s < 4; s++) *reinterpret_cast<double2 *>(&As[(t >> 6) + 4 * s][2 * cr]) = pa[s];
+#pragma unroll
+    for (int s = 0; s < 8; s++) Bs[(t >> 7) + 2 * s][t & 127] = pb[s];
     __syncthreads();
-    // v_mfma_f64_16x16x4f64 returns D[(lane/16) + 4v][lane%16] in register v.
-    // Feeding MFMA row i from tile row 4(i%4) + i/4 makes register v of lane l
-    // hold tile row 4(l/16) + v, so each lane ends with two whole complex rows.
-    const int arow = 4 * (lane & 3) + ((lane & 15) >> 2);
+    if (k0 + TG_KC64 < K) fetch(k0 + TG_KC64);
 #pragma unroll
     for (int ks = 0; ks < TG_KC64 / 4; ks++) {
       const int kr = ks * 4 + (lane >> 4);
@@ -728,7 +736,6 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
         for (int rb = 0; rb < 2; rb++) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb][cb], 0, 0, 0);
       }
     }
-    __syncthreads();
   }
   // lane l holds tile rows 4(l/16)+0..3 (complex rows 2(l/16), 2(l/16)+1) of column l%16
 #pragma unroll
@@ -798,60 +805,85 @@ __global__ __launch_bounds__(256) void k_theta_bf16(ThetaArgs P) {
 #pragma unroll
     for (int b = 0; b < NCB; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < K; k0 += TG_KC16) {
-    // A chunk: thread owns complex row cr and 8 consecutive k
-    {
-      const int cr = t & 63;
-      const int kg = t >> 6;  // 0..3
-      const int r = row0 + cr;
-      float re[8], im[8];
+  // The next K chunk is fetched into registers while the current one is on
+  // the matrix cores: A as 8 complex values (row cr, k = kg*8..kg*8+7) with
+  // the phase prologue applied, B as bf16 vectors.
+  const int cr = t & 63;
+  const int kg = t >> 6;  // 0..3
+  const int r = row0 + cr;
+  constexpr int BV = (NS * NCB * 16 * 2 + 255) / 256;  // B vectors (2 x bf8v) per thread
+  float2 pa[8];
+  bf8v pb[BV][2];
+  auto fetch = [&](int k0) {
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int k = k0 + kg * 8 + e;
-        float2 v = make_float2(0.f, 0.f);
-        if (r < R && k < K) {
-          v = A[(size_t)r + (size_t)R * k];
-          if (PRO) {
-            v = cmul(v, expV[P.dcol_vidx[acol0 + k] + r]);
-            if (PRO == 2) {
-              const float d = P.d1f[r % P.n1] * P.d2f[r / P.n1];
-              v.x *= d;
-              v.y *= d;
-            }
+    for (int e = 0; e < 8; e++) {
+      const int k = k0 + kg * 8 + e;
+      float2 v = make_float2(0.f, 0.f);
+      if (r < R && k < K) {
+        v = A[(size_t)r + (size_t)R * k];
+        if (PRO) {
+          v = cmul(v, expV[P.dcol_vidx[acol0 + k] + r]);
+          if (PRO == 2) {
+            const float d = P.d1f[r % P.n1] * P.d2f[r / P.n1];
+            v.x *= d;
+            v.y *= d;
           }
         }
-        re[e] = v.x;
-        im[e] = v.y;
       }
+      pa[e] = v;
+    }
 #pragma unroll
-      for (int p = 0; p < NS; p++) {
-        bf8v hr, hi;
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          const __bf16 a = f2bf(re[e]);
-          const __bf16 b = f2bf(im[e]);
-          hr[e] = a;
-          hi[e] = b;
-          re[e] -= (float)a;
-          im[e] -= (float)b;
-        }
-        *reinterpret_cast<bf8v *>(&As[p][2 * cr][kg * 8]) = hr;
-        *reinterpret_cast<bf8v *>(&As[p][2 * cr + 1][kg * 8]) = hi;
+    for (int i = 0; i < BV; i++) {
+      const int u = t + i * 256;
+      if (u < NS * NCB * 32) {
+        const int p = u / (NCB * 32);
+        const int rem = u % (NCB * 32);
+        const __bf16 *src = Bsplit + ((size_t)p * P.Npad + col0 + (rem >> 1)) * P.Kpad + k0 + (rem & 1) * 16;
+        pb[i][0] = *reinterpret_cast<const bf8v *>(src);
+        pb[i][1] = *reinterpret_cast<const bf8v *>(src + 8);
       }
     }
-    // B chunk: NCB*16 columns x 32 k per piece
-    for (int u = t; u < NS * NCB * 16 * 2; u += 256) {
-      const int p = u / (NCB * 32);
-      const int rem = u % (NCB * 32);
-      const int col = rem >> 1;
-      const int half = rem & 1;
-      const __bf16 *src = Bsplit + ((size_t)p * P.Npad + col0 + col) * P.Kpad + k0 + half * 16;
-      const bf8v b0 = *reinterpret_cast<const bf8v *>(src);
-      const bf8v b1 = *reinterpret_cast<const bf8v *>(src + 8);
-      *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16]) = b0;
-      *reinterpret_cast<bf8v *>(&Bs[p][col][half * 16 + 8]) = b1;
+  };
+  auto stash = [&]() {
+    float re[8], im[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      re[e] = pa[e].x;
+      im[e] = pa[e].y;
     }
+#pragma unroll
+    for (int p = 0; p < NS; p++) {
+      bf8v hr, hi;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const __bf16 a = f2bf(re[e]);
+        const __bf16 b = f2bf(im[e]);
+        hr[e] = a;
+        hi[e] = b;
+        re[e] -= (float)a;
+        im[e] -= (float)b;
+      }
+      *reinterpret_cast<bf8v *>(&As[p][2 * cr][kg * 8]) = hr;
+      *reinterpret_cast<bf8v *>(&As[p][2 * cr + 1][kg * 8]) = hi;
+    }
+#pragma unroll
+    for (int i = 0; i < BV; i++) {
+      const int u = t + i * 256;
+      if (u < NS * NCB * 32) {
+        const int p = u / (NCB * 32);
+        const int rem = u % (NCB * 32);
+        *reinterpret_cast<bf8v *>(&Bs[p][rem >> 1][(rem & 1) * 16]) = pb[i][0];
+        *reinterpret_cast<bf8v *>(&Bs[p][rem >> 1][(rem & 1) * 16 + 8]) = pb[i][1];
+      }
+    }
+  };
+
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += TG_KC16) {
+    __syncthreads();  // the previous chunk's fragments have been read
+    stash();
     __syncthreads();
+    if (k0 + TG_KC16 < K) fetch(k0 + TG_KC16);
     bf8v a[NS][2];
 #pragma unroll
     for (int p = 0; p < NS; p++)
@@ -875,12 +907,11 @@ __global__ __launch_bounds__(256) void k_theta_bf16(ThetaArgs P) {
             acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p][rb], b[q], acc[rb][cb], 0, 0, 0);
         }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int rb = 0; rb < 2; rb++) {
     const int rr = wave * 32 + rb * 16 + 4 * (lane >> 4);
-    const int r = row0 + rr / 2;
+    const int rw = row0 + rr / 2;
 #pragma unroll
     for (int cb = 0; cb < NCB; cb++) {
       const int col = col0 + cb * 16 + (lane & 15);
@@ -888,20 +919,20 @@ __global__ __launch_bounds__(256) void k_theta_bf16(ThetaArgs P) {
       float2 c0 = make_float2(acc[rb][cb][0], acc[rb][cb][1]);
       float2 c1 = make_float2(acc[rb][cb][2], acc[rb][cb][3]);
       if (EPI) {
-        if (r < R) {
-          const float d = P.d1f[r % P.n1] * P.d2f[r / P.n1];
+        if (rw < R) {
+          const float d = P.d1f[rw % P.n1] * P.d2f[rw / P.n1];
           c0.x *= d;
           c0.y *= d;
         }
-        if (r + 1 < R) {
-          const float d = P.d1f[(r + 1) % P.n1] * P.d2f[(r + 1) / P.n1];
+        if (rw + 1 < R) {
+          const float d = P.d1f[(rw + 1) % P.n1] * P.d2f[(rw + 1) / P.n1];
           c1.x *= d;
           c1.y *= d;
         }
       }
-      const size_t x = (size_t)r + (size_t)R * col;
-      if (r < R) Cm[x] = c0;
-      if (r + 1 < R) Cm[x + 1] = c1;
+      const size_t x = (size_t)rw + (size_t)R * col;
+      if (rw < R) Cm[x] = c0;
+      if (rw + 1 < R) Cm[x + 1] = c1;
     }
   }
 }
