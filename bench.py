@@ -72,6 +72,22 @@ def cpu_baseline(prob, seconds_target=15.0):
                       % (n, dt)}
 
 
+def calibrate_copy(mib=1024, reps=5):
+    """Copies `mib` MiB device to device `reps` times: the reference reading
+    for the FETCH_SIZE / WRITE_SIZE counters (each launch moves exactly
+    mib MiB in and mib MiB out)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_MiB": mib, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +95,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a device-to-device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the propagator on a single MI355X; "
               "J blocks would shard across GPUs by J (see DESIGN.md)", file=sys.stderr)
@@ -91,7 +112,8 @@ def main():
     wall_m, step_m, _ = timed_run(qd, prob, "mixed", "fused", args.steps, args.warmup)
     _, _, kern_m = timed_run(qd, prob, "mixed", "fused", max(20, args.steps // 4), 5, profile=1)
     wall_p, step_p, _ = timed_run(qd, prob, "fp64", "plain", max(20, args.steps // 2), 5)
-    wall_f, step_f, kern_f = timed_run(qd, prob, "fp64", "fused", max(20, args.steps // 2), 5)
+    wall_f, step_f, _ = timed_run(qd, prob, "fp64", "fused", max(20, args.steps // 2), 5)
+    _, _, kern_f = timed_run(qd, prob, "fp64", "fused", max(20, args.steps // 4), 5, profile=1)
 
     value = npt / (step_m * 1e-3)
     plain = npt / (step_p * 1e-3)

@@ -186,6 +186,9 @@ struct qd_handle {
   int *unit_cols = nullptr;
   void *umat = nullptr;
   int nbundle = 0, rp_tpl = 16, rp_ml = 8;
+  // matrix-stationary mixed theta kernel: k-steps (0: tiled kernel), tiles per work group
+  int tmx_ks = 0, tmx_tpw = 0;
+  size_t tmx_lds = 0;
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
   FftPlanArg planR{}, planr{};
@@ -354,6 +357,23 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   return QD_OK;
 }
 
+// matrix-stationary mixed theta kernel for KS k-steps (KP = NP = 32 KS)
+const void *theta_mx_kernel(int ks, int pro, int epi) {
+#define QD_TMX(KS_)                                                                    \
+  if (ks == KS_) {                                                                     \
+    if (pro == 0 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 0>); \
+    if (pro == 1 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 1, 0>); \
+    if (pro == 2 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 2, 0>); \
+    if (pro == 0 && epi == 1) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 1>); \
+  }
+  QD_TMX(1)
+  QD_TMX(2)
+  QD_TMX(3)
+  QD_TMX(4)
+#undef QD_TMX
+  return nullptr;
+}
+
 int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   ThetaArgs P;
   P.A = h->prec == QD_PREC_FP64 ? (const void *)(fwd ? h->psi64 : h->lpsi64)
@@ -375,6 +395,16 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   P.d1f = h->d1f;
   P.d2f = h->d2f;
   P.n1 = h->n1;
+  if (h->tmx_ks) {
+    P.tpw = h->tmx_tpw;
+    const void *fn = theta_mx_kernel(h->tmx_ks, pro, epi);
+    if (!fn) return fail(QD_ERR_ARG, "bad theta variant");
+    const long long ntiles = (h->n12 + 15) / 16;
+    dim3 grid((unsigned)((ntiles + h->tmx_tpw - 1) / h->tmx_tpw), 1, h->nb);
+    void *args[] = {&P};
+    QD_HIP(hipLaunchKernel(fn, grid, dim3(TMX_THREADS), args, h->tmx_lds, h->stream));
+    return QD_OK;
+  }
   const int rows = (int)((h->n12 + TG_ROWS - 1) / TG_ROWS);
   dim3 grid(rows, h->col_tiles, h->nb);
   if (h->prec == QD_PREC_FP64) {
@@ -611,6 +641,43 @@ int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
   return QD_OK;
 }
 
+// LDS limit and work split of the matrix-stationary theta kernel.  A work
+// group loads its block's matrix once and then streams tpw 16-row tiles; tpw
+// minimises (work-group rounds) x (tpw + matrix load), where a round is one
+// wave of resident work groups over the whole chip.
+int setup_theta_mx(qd_handle *h) {
+  int ncu = 0;
+  QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
+  int per_cu = 1;
+  for (int pro = 0; pro <= 2; pro++)
+    for (int epi = 0; epi <= 1; epi++) {
+      const void *fn = theta_mx_kernel(h->tmx_ks, pro, epi);
+      if (!fn) continue;
+      QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->tmx_lds));
+      int nb = 0;
+      QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, TMX_THREADS, h->tmx_lds));
+      if (nb < 1) return fail(QD_ERR_DEVICE, "theta kernel does not fit on a compute unit");
+      per_cu = pro + epi == 0 ? nb : std::min(per_cu, nb);
+    }
+  const long long slots = (long long)std::max(1, ncu) * per_cu;
+  const long long ntiles = (h->n12 + 15) / 16;
+  const long long nb = h->nb;
+  double best = 1e300;
+  int best_tpw = 1;
+  const double load_cost = 0.25 * h->tmx_ks;  // matrix load in tile units
+  for (long long tpw = 1; tpw <= ntiles; tpw++) {
+    const long long wgs = nb * ((ntiles + tpw - 1) / tpw);
+    const long long rounds = (wgs + slots - 1) / slots;
+    const double cost = (double)rounds * ((double)std::max<long long>(tpw, TMX_THREADS / 64 / 2) + load_cost);
+    if (cost < best - 1e-9) {
+      best = cost;
+      best_tpw = (int)tpw;
+    }
+  }
+  h->tmx_tpw = best_tpw;
+  return QD_OK;
+}
+
 int create_batch(qd_handle *h, const BatchSpec &S) {
   const qd_problem *p = S.base;
   const int n1 = p->r1.n, n2 = p->r2.n;
@@ -763,6 +830,15 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   h->col_tiles = h->mat_Npad / colw;
   const int kc = h->prec == QD_PREC_FP64 ? TG_KC64 : TG_KC16;
   h->mat_Kpad = (maxn + kc - 1) / kc * kc;
+  // the matrix-stationary kernel holds a whole (32 ks)^2 matrix in LDS and
+  // addresses one block's columns with 32-bit byte offsets
+  h->tmx_ks = 0;
+  if (h->prec == QD_PREC_MIXED && h->nsplit == 3 && maxn <= 128 &&
+      (double)((maxn + 31) / 32 * 32) * (double)h->n12 * 8.0 < 2147483648.0) {
+    h->tmx_ks = (maxn + 31) / 32;
+    h->mat_Npad = h->mat_Kpad = 32 * h->tmx_ks;
+    h->tmx_lds = (size_t)3 * 32 * h->tmx_ks * (32 * h->tmx_ks + 8) * 2;
+  }
 
   // ---- everything above is host-side validation; device work starts here ----
   QD_HIP(hipSetDevice(h->dev));
@@ -782,6 +858,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     const void *fn = h->prec == QD_PREC_FP64 ? rline_kernel<double>(n2, h->rl_lpb) : rline_kernel<float>(n2, h->rl_lpb);
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->rl_lds));
   }
+  if (h->tmx_ks) QD_RC(setup_theta_mx(h));
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));
   QD_RC(upload(&h->dcol_w, dcol_w.data(), dcol_w.size()));

@@ -655,6 +655,7 @@ struct ThetaArgs {
   const double *d1, *d2;
   const float *d1f, *d2f;
   int n1;
+  int tpw;                     // 16-row tiles per work group (matrix-stationary kernel)
 };
 
 template <int PRO, int EPI>
@@ -935,6 +936,161 @@ __global__ __launch_bounds__(256) void k_theta_bf16(ThetaArgs P) {
       if (rw + 1 < R) Cm[x + 1] = c1;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// theta transform, mixed precision, matrix-stationary (n_theta <= 128)
+// ---------------------------------------------------------------------------
+// The product is taken transposed, C^T[n][row] = sum_k M[k][n] psi[row][k]:
+// the theta matrix (three bf16 pieces, [piece][n][k]) is the MFMA A operand
+// and stays in LDS for the whole work group; psi goes from HBM straight into
+// registers, one k-step (32 columns) ahead of the matrix cores, and is split
+// into bf16 pieces there.  Lane l owns grid row (l % 16) of a 16-row tile, so
+// its output column n is a 128-byte coalesced store over the 16 lanes, and the
+// phase / dump factors are per-lane scalars.
+// Work: a work group takes tiles [blockIdx.x * tpw, +tpw) of block blockIdx.z;
+// its waves take every W-th tile.
+// Raw buffer access (gfx9 descriptor, stride 0).  An offset at or beyond the
+// descriptor's byte count reads as zero and drops a store, which makes the
+// edge tiles branch-free: masked elements get offset TM_OOB.
+constexpr unsigned TM_OOB = 0x80000000u;
+constexpr int TMX_THREADS = 512;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_ld_f2(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void buf_st_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x2 v;
+  v.x = __float_as_uint(x.x);
+  v.y = __float_as_uint(x.y);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
+}
+
+template <int KS, int NT, int PRO, int EPI>
+__global__ __launch_bounds__(512) void k_theta_mx(ThetaArgs P) {
+  extern __shared__ __align__(16) unsigned char tm_lds[];
+  constexpr int KP = KS * 32, NP = NT * 16, LD = KP + 8;
+  __bf16 *Ms = reinterpret_cast<__bf16 *>(tm_lds);  // [3][NP][LD]
+  const BlockDesc bd = P.blocks[blockIdx.z];
+  const int R = P.R;
+  const int K = P.fwd ? bd.nd : bd.nf;
+  const int N = P.fwd ? bd.nf : bd.nd;
+  const size_t acol0 = P.fwd ? bd.dcol0 : bd.fcol0;
+  const size_t ccol0 = P.fwd ? bd.fcol0 : bd.dcol0;
+  // block-local descriptors: psi in, phase columns (contiguous per block), out
+  const __amdgpu_buffer_rsrc_t rA = buf_rsrc(reinterpret_cast<const float2 *>(P.A) + acol0 * R, (unsigned)K * R * 8u);
+  const __amdgpu_buffer_rsrc_t rV =
+      buf_rsrc(PRO ? reinterpret_cast<const float2 *>(P.expV) + P.dcol_vidx[acol0] : nullptr, PRO ? (unsigned)K * R * 8u : 0u);
+  const __amdgpu_buffer_rsrc_t rC = buf_rsrc(reinterpret_cast<float2 *>(P.C) + ccol0 * R, (unsigned)N * R * 8u);
+  {
+    const __bf16 *Mg = reinterpret_cast<const __bf16 *>(P.B) + (size_t)bd.mat * P.mat_stride;
+    constexpr int QV = KP / 8;  // bf8v per matrix row
+    for (int u = threadIdx.x; u < 3 * NP * QV; u += blockDim.x) {
+      const int pn = u / QV, kq = u % QV;
+      *reinterpret_cast<bf8v *>(&Ms[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&Mg[(size_t)pn * KP + kq * 8]);
+    }
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int rl = lane & 15, kg = lane >> 4;
+  const int ntiles = (R + 15) >> 4;
+  const int t1 = min(ntiles, (int)(blockIdx.x + 1) * P.tpw);
+  float2 ra[8], rv[8];
+  auto fetch = [&](int tile, int s) {
+    const int row = tile * 16 + rl;
+    const int k0 = s * 32 + kg * 8;
+    // a row past the end starts beyond the descriptor (K * R * 8 < 2^31)
+    const unsigned base = (row < R ? (unsigned)row * 8u : TM_OOB) + (unsigned)(R * k0) * 8u;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const unsigned off = k0 + e < K ? base + (unsigned)(R * e) * 8u : TM_OOB;
+      ra[e] = buf_ld_f2(rA, off);
+      if (PRO) rv[e] = buf_ld_f2(rV, off);
+    }
+  };
+  int tile = blockIdx.x * P.tpw + wave;
+  if (tile < t1) fetch(tile, 0);
+  for (; tile < t1; tile += W) {
+    const int row = tile * 16 + rl;
+    float dfac = 1.f;
+    if ((PRO == 2 || EPI) && row < R) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
+    f4v acc[2][NT];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < NT; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int s = 0; s < KS; s++) {
+      bf8v bre[3], bim[3];
+      {
+        float re[8], im[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          float2 v = ra[e];
+          if (PRO) v = cmul(v, rv[e]);
+          if (PRO == 2) {
+            v.x *= dfac;
+            v.y *= dfac;
+          }
+          re[e] = v.x;
+          im[e] = v.y;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const __bf16 a = f2bf(re[e]);
+            const __bf16 b = f2bf(im[e]);
+            bre[q][e] = a;
+            bim[q][e] = b;
+            re[e] -= (float)a;
+            im[e] -= (float)b;
+          }
+      }
+      {
+        const bool last = s + 1 == KS;
+        const int ft = last ? tile + W : tile;
+        if (ft < t1) fetch(ft, last ? 0 : s + 1);
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        bf8v a[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++) a[p] = *reinterpret_cast<const bf8v *>(&Ms[(p * NP + nt * 16 + rl) * LD + s * 32 + kg * 8]);
+        // small products first
+#pragma unroll
+        for (int d = 2; d >= 0; d--)
+#pragma unroll
+          for (int p = 0; p <= d; p++) {
+            acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bre[d - p], acc[0][nt], 0, 0, 0);
+            acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bim[d - p], acc[1][nt], 0, 0, 0);
+          }
+      }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int n = nt * 16 + 4 * kg + v;
+        float2 c = make_float2(acc[0][nt][v], acc[1][nt][v]);
+        if (EPI) {
+          c.x *= dfac;
+          c.y *= dfac;
+        }
+        buf_st_f2(c, rC, (row < R && n < N) ? (unsigned)(row + R * n) * 8u : TM_OOB);
+      }
+  }
+}
+
+template <int KS, int NT>
+constexpr size_t theta_mx_lds() {
+  return (size_t)3 * NT * 16 * (KS * 32 + 8) * 2;
 }
 
 // ---------------------------------------------------------------------------
