@@ -28,6 +28,7 @@
 
 #include "../../include/qd.h"
 #include "qd_fft.hpp"
+#include "qd_rfft.hpp"
 #include "qd_kernels.hpp"
 #include "qd_tables.hpp"
 
@@ -188,6 +189,9 @@ struct qd_handle {
   int nbundle = 0, rp_tpl = 16, rp_ml = 8;
   // matrix-stationary mixed theta kernel: k-steps (0: tiled kernel), tiles per work group
   int tmx_ks = 0, tmx_tpw = 0;
+  // register-FFT passes (fused forward / kinetic / inverse with phases)
+  bool rp_rf = false, rl_rf = false;
+  int rp_lw = 0;
   size_t tmx_lds = 0;
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
@@ -298,6 +302,54 @@ const void *rline_kernel(int n, int lpb) {
   }
 }
 
+// register-FFT passes: lengths with a factorisation N1 x N2 (rf_n1)
+#define QD_RF_LENGTHS(X) X(24, 8) X(32, 8) X(48, 8) X(64, 8) X(96, 16) X(128, 16) X(192, 16) X(256, 16) X(384, 32) X(512, 32)
+template <typename T, int N1, int N2>
+const void *rpass_rf_fn(bool mix, int lw, size_t *lds) {
+  constexpr int LW0 = 256 / N1;
+  if (!mix) {
+    if (lw != LW0) return nullptr;
+    *lds = rpass_rf_lds<T, N1, N2, false, LW0>();
+    return (const void *)k_rpass_rf<T, N1, N2, false, LW0>;
+  }
+  if (lw == LW0) {
+    *lds = rpass_rf_lds<T, N1, N2, true, LW0>();
+    return (const void *)k_rpass_rf<T, N1, N2, true, LW0>;
+  }
+  if (lw == 2 * LW0) {  // 512 threads: mixing groups of up to 2 * LW0 lines
+    *lds = rpass_rf_lds<T, N1, N2, true, 2 * LW0>();
+    return (const void *)k_rpass_rf<T, N1, N2, true, 2 * LW0>;
+  }
+  return nullptr;
+}
+template <typename T>
+const void *rpass_rf_kernel(int n, bool mix, int lw, size_t *lds) {
+  switch (n) {
+#define QD_CASE(N, N1) \
+  case N:              \
+    return rpass_rf_fn<T, N1, N / N1>(mix, lw, lds);
+    QD_RF_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+template <typename T>
+const void *rline_rf_kernel(int n, size_t *lds, int *lpb) {
+  switch (n) {
+#define QD_CASE(N, N1)                             \
+  case N:                                          \
+    *lds = rline_rf_lds<T, N1, N / N1>();          \
+    *lpb = 256 / N1;                               \
+    return (const void *)k_rline_rf<T, N1, N / N1>;
+    QD_RF_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+constexpr int QD_RF_FLAGS = FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV;
+
 // ---- launches ----
 template <typename T>
 int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
@@ -319,6 +371,18 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.flags = flags;
   if (h->has_mix && (flags & FFT_PHASE)) a.flags |= FFT_MIX;
   a.plan = h->planR;
+  a.nfcol = (int)h->nfcol;
+  if (h->rp_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
+    size_t lds = 0;
+    const void *fn = rpass_rf_kernel<T>(h->n1, h->has_mix, h->rp_lw, &lds);
+    void *args[] = {&a};
+    const int lw = h->rp_lw;
+    const int nthreads = lw * rf_n1(h->n1);
+    const unsigned nblk = h->has_mix ? (unsigned)h->nbundle : (unsigned)(((long long)h->nfcol * h->n2 + lw - 1) / lw);
+    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(nthreads), args, lds, h->stream));
+    QD_HIP(hipGetLastError());
+    return QD_OK;
+  }
   const void *fn = h->rp_ct ? rpass_kernel<T>(h->n1, h->rp_ml, h->has_mix) : nullptr;
   if (fn) {
     void *args[] = {&a};
@@ -346,6 +410,16 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.nlines = h->n1 * h->nfcol;
   a.flags = flags;
   a.plan = h->planr;
+  if (h->rl_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
+    size_t lds = 0;
+    int lpb = 0;
+    const void *fn = rline_rf_kernel<T>(h->n2, &lds, &lpb);
+    void *args[] = {&a};
+    const unsigned nblk = (unsigned)(((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
+    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, h->stream));
+    QD_HIP(hipGetLastError());
+    return QD_OK;
+  }
   const void *fn = h->rl_ct ? rline_kernel<T>(h->n2, h->rl_lpb) : nullptr;
   if (fn) {
     void *args[] = {&a};
@@ -823,6 +897,27 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     h->rl_lds = h->rl_ct ? 3 * (size_t)n2 * tabsz + (size_t)n2 * (h->rl_lpb + 1) * elem
                          : 3 * (size_t)n2 * tabsz + (size_t)h->rl_lpb * n2 * elem;
   }
+  // register-FFT passes for the fused step (other flag sets keep the
+  // Stockham kernels above)
+  h->rp_rf = h->rl_rf = false;
+  if (rf_length(n1)) {
+    const int n1f = rf_n1(n1);
+    int lw = 256 / n1f;
+    if (h->has_mix) lw = std::max(lw, h->rp_ml);  // a bundle holds up to rp_ml lines
+    size_t lds = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, h->has_mix, lw, &lds)
+                                             : rpass_rf_kernel<float>(n1, h->has_mix, lw, &lds);
+    if (fn && lds <= 160 * 1024 && lw * n1f <= 1024) {
+      h->rp_rf = true;
+      h->rp_lw = lw;
+    }
+  }
+  if (rf_length(n2) && n1 % (256 / rf_n1(n2)) == 0) {
+    size_t lds = 0;
+    int lpb = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, &lds, &lpb) : rline_rf_kernel<float>(n2, &lds, &lpb);
+    h->rl_rf = fn && lds <= 160 * 1024;
+  }
   // theta matrices: padded shapes shared by all sets
   h->ncb = (h->prec == QD_PREC_MIXED && maxn <= 64) ? 4 : 8;
   const int colw = h->prec == QD_PREC_FP64 ? TG_COLS : h->ncb * 16;
@@ -859,6 +954,18 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->rl_lds));
   }
   if (h->tmx_ks) QD_RC(setup_theta_mx(h));
+  if (h->rp_rf) {
+    size_t lds = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, h->has_mix, h->rp_lw, &lds)
+                                             : rpass_rf_kernel<float>(n1, h->has_mix, h->rp_lw, &lds);
+    QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  if (h->rl_rf) {
+    size_t lds = 0;
+    int lpb = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, &lds, &lpb) : rline_rf_kernel<float>(n2, &lds, &lpb);
+    QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));
   QD_RC(upload(&h->dcol_w, dcol_w.data(), dcol_w.size()));

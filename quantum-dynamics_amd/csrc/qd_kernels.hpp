@@ -386,6 +386,7 @@ struct RPassArgs {
   int tpl;
   int flags;
   FftPlanArg plan;
+  int nfcol;                           // FBR columns (lines = nfcol * n2)
 };
 
 template <typename T>

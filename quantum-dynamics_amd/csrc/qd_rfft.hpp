@@ -1,0 +1,415 @@
+// qd_rfft.hpp -- register-resident FFT passes (forward, kinetic, inverse).
+//
+// A line of N = N1 * N2 points is split x[a + N1 m], a < N1, m < N2:
+//   stage A   thread a:  length-N2 DFT over m -> k2, times W_N^(a k2)
+//   exchange  (LDS)
+//   stage B   thread b:  length-N1 DFT over a -> k1, giving X[b + N2 k1];
+//             times the kinetic factor; inverse length-N1 DFT -> c;
+//             times W_N^(-b c)
+//   exchange  (LDS)
+//   stage B'  thread a:  inverse length-N2 DFT -> y[a + N1 c1]
+// so the forward transform, the kinetic phase and the inverse transform of a
+// line cost two LDS exchanges, and every thread ends with the elements it
+// loaded (the same global addresses, which keeps the store coalesced).
+// The small DFTs are unrolled at compile time with exact constants; in FP32
+// every non-trivial constant and table twiddle is applied as hi + lo (the
+// rounding of a single-float twiddle biases |W| and, applied coherently every
+// step, the norm).
+#pragma once
+
+#include <utility>
+
+#include "qd_kernels.hpp"
+
+namespace qd {
+
+// ---- compile-time loop and trigonometry ------------------------------------
+template <int... Is, typename F>
+__device__ __forceinline__ void sfor_impl(std::integer_sequence<int, Is...>, F &&f) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F &&f) {
+  sfor_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+constexpr double rf_pi = 3.14159265358979323846264338327950288;
+constexpr double rf_sin_small(double x) {  // |x| <= pi/4, Taylor series
+  double t = x, s = x;
+  for (int n = 1; n < 14; n++) {
+    t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double rf_cos_small(double x) {
+  double t = 1.0, s = 1.0;
+  for (int n = 1; n < 14; n++) {
+    t *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    s += t;
+  }
+  return s;
+}
+// cos or sin of 2 pi e / S; the octant of the angle is found on the integers
+// so the series only sees arguments in [0, pi/4].
+constexpr double rf_trig(int e, int S, bool want_sin) {
+  e %= S;
+  if (e < 0) e += S;
+  const long long f = 8LL * e;        // angle = (pi/4) f / S
+  const int q = (int)(f / S);         // octant 0..7
+  const long long rn = f - (long long)q * S;
+  double c = 0, s = 0;
+  int Q = 0;
+  if (q % 2 == 0) {  // angle = (q/2)(pi/2) + r
+    const double r = (rf_pi / 4) * (double)rn / (double)S;
+    c = rf_cos_small(r);
+    s = rf_sin_small(r);
+    Q = q / 2;
+  } else {           // angle = ((q+1)/2)(pi/2) - r'
+    const double r = (rf_pi / 4) * (double)(S - rn) / (double)S;
+    c = rf_cos_small(r);
+    s = -rf_sin_small(r);
+    Q = (q + 1) / 2;
+  }
+  double cc = c, ss = s;
+  switch (Q & 3) {
+    case 1: cc = -s; ss = c; break;
+    case 2: cc = -c; ss = -s; break;
+    case 3: cc = s; ss = -c; break;
+    default: break;
+  }
+  return want_sin ? ss : cc;
+}
+
+// x * W_S^E with W = exp(-2 pi i / S) (FWD) or its conjugate.
+template <typename T, int S, int E, bool FWD>
+__device__ __forceinline__ typename cx_of<T>::type rf_twc(typename cx_of<T>::type x) {
+  typedef typename cx_of<T>::type C;
+  constexpr int e = ((E % S) + S) % S;
+  if constexpr (e == 0) {
+    return x;
+  } else if constexpr (2 * e == S) {
+    C r;
+    r.x = -x.x;
+    r.y = -x.y;
+    return r;
+  } else if constexpr (4 * e == S || 4 * e == 3 * S) {
+    // exp(-i pi/2) = -i for e = S/4 forward
+    constexpr bool minus_i = (4 * e == S) == FWD;
+    C r;
+    if (minus_i) {
+      r.x = x.y;
+      r.y = -x.x;
+    } else {
+      r.x = -x.y;
+      r.y = x.x;
+    }
+    return r;
+  } else {
+    constexpr double wr = rf_trig(e, S, false);
+    constexpr double wi = FWD ? -rf_trig(e, S, true) : rf_trig(e, S, true);
+    C r;
+    if constexpr (sizeof(T) == 8) {
+      r.x = x.x * wr - x.y * wi;
+      r.y = x.x * wi + x.y * wr;
+    } else {
+      constexpr float hr = (float)wr, hi = (float)wi;
+      constexpr float lr = (float)(wr - (double)hr), li = (float)(wi - (double)hi);
+      const float tx = x.x * lr - x.y * li;
+      const float ty = x.x * li + x.y * lr;
+      r.x = fmaf(x.x, hr, fmaf(-x.y, hi, tx));
+      r.y = fmaf(x.x, hi, fmaf(x.y, hr, ty));
+    }
+    return r;
+  }
+}
+
+// In-place DFT of S points held in registers (natural order in and out),
+// decimation in time with the radix preference 4, 2, 3, 5.
+template <typename T, int S, bool FWD>
+__device__ __forceinline__ void rdft(typename cx_of<T>::type *x) {
+  typedef typename cx_of<T>::type C;
+  if constexpr (S == 2 || S == 3 || S == 4 || S == 5) {
+    dft_small<C, T>(x, S, FWD ? (T)-1 : (T)1);
+  } else if constexpr (S > 5) {
+    constexpr int r = S % 4 == 0 ? 4 : S % 2 == 0 ? 2 : S % 3 == 0 ? 3 : 5;
+    constexpr int m = S / r;
+    static_assert(m * r == S, "length must factor into 2, 3, 5");
+    C sub[r][m];
+    sfor<r>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      sfor<m>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        sub[j][i] = x[j + r * i];
+      });
+      rdft<T, m, FWD>(sub[j]);
+    });
+    sfor<m>([&](auto Kc) {
+      constexpr int k = decltype(Kc)::value;
+      C y[r];
+      sfor<r>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        y[j] = rf_twc<T, S, j * k, FWD>(sub[j][k]);
+      });
+      dft_small<C, T>(y, r, FWD ? (T)-1 : (T)1);
+      sfor<r>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        x[k + m * q] = y[q];
+      });
+    });
+  }
+}
+
+// Factorisation used for a line length: N1 threads per line (a divisor of 256),
+// N2 <= N1 points per thread.
+constexpr int rf_n1(int n) {
+  return n == 512 ? 32 : n == 384 ? 32 : n == 256 ? 16 : n == 192 ? 16 : n == 128 ? 16 : n == 96 ? 16
+       : n == 64 ? 8 : n == 48 ? 8 : n == 32 ? 8 : n == 24 ? 8 : 0;
+}
+constexpr bool rf_length(int n) { return rf_n1(n) != 0; }
+
+// Stages A .. B' of one line (see the file comment).  v: the thread's N2
+// points on entry and exit; xidx(n, k): LDS slot of exchange element (n, k)
+// of this line; twa[a * (N2 + 1) + k] = W_N^(a k); kin[k]: kinetic factor.
+template <typename T, int N1, int N2, typename XI>
+__device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool act, typename cx_of<T>::type *xb,
+                                        XI xidx, const typename tab_of<T>::type *twa,
+                                        const typename tab_of<T>::type *kin) {
+  typedef typename cx_of<T>::type C;
+  constexpr int XS = N2 + 1;
+  if (act) {
+    rdft<T, N2, true>(v);
+#pragma unroll
+    for (int k = 1; k < N2; k++) v[k] = tmul(v[k], twa[a * XS + k], false);
+#pragma unroll
+    for (int k = 0; k < N2; k++) xb[xidx(a, k)] = v[k];
+  }
+  __syncthreads();
+  if (act && a < N2) {
+    C u[N1];
+#pragma unroll
+    for (int n = 0; n < N1; n++) u[n] = xb[xidx(n, a)];
+    rdft<T, N1, true>(u);
+#pragma unroll
+    for (int k = 0; k < N1; k++) u[k] = tmul(u[k], kin[a + N2 * k], false);
+    rdft<T, N1, false>(u);
+#pragma unroll
+    for (int c = 1; c < N1; c++) u[c] = tmul(u[c], twa[c * XS + a], true);
+#pragma unroll
+    for (int c = 0; c < N1; c++) xb[xidx(c, a)] = u[c];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N2; k++) v[k] = xb[xidx(a, k)];
+    rdft<T, N2, false>(v);
+  }
+}
+
+template <int N1, int N2>
+constexpr size_t rf_table_bytes(size_t tabsz) {
+  return ((size_t)N1 * (N2 + 1) + (size_t)N1 * N2) * tabsz;
+}
+
+// Twiddle and kinetic tables into LDS.
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void rf_tables(typename tab_of<T>::type *twa, typename tab_of<T>::type *kin,
+                                          const typename tab_of<T>::type *tw, const typename tab_of<T>::type *kg) {
+  constexpr int N = N1 * N2, XS = N2 + 1;
+  for (int e = threadIdx.x; e < N; e += blockDim.x) {
+    const int aa = e / N2, kk = e % N2;
+    twa[aa * XS + kk] = tw[aa * kk];
+    kin[e] = kg[e];
+  }
+}
+
+// R pass: lines (column, r) of N = n1 contiguous points; LW lines per work
+// group of LW * N1 threads (LW >= the largest mixing group).  Without K mixing each thread loads its points straight from
+// HBM; with mixing the lines of a bundle are staged in LDS, mixed into the l
+// basis, transformed, and mixed back.
+template <typename T, int N1, int N2, bool MIXT, int LW>
+__global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
+  typedef typename cx_of<T>::type C;
+  typedef typename tab_of<T>::type TB;
+  constexpr int N = N1 * N2, NTH = LW * N1, XS = N2 + 1, LX = N1 * XS;
+  constexpr int SP = N + N / 32 + 1;  // staged line stride
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
+  TB *s_kin = s_twa + N1 * XS;
+  C *xch = reinterpret_cast<C *>(s_kin + N);  // [LW][LX]
+  C *stg = xch + LW * LX;                       // [LW][SP], mixing kernels only
+  const int t = threadIdx.x;
+  const int a = t % N1, L = t / N1;
+  rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
+  C v[N2];
+  C *xb = xch + L * LX;
+  auto xidx = [](int n, int k) { return n * XS + k; };
+  if constexpr (!MIXT) {
+    const long long nlines = (long long)A.nfcol * A.n2;
+    const long long line = (long long)blockIdx.x * LW + L;
+    const bool act = line < nlines;
+    const int lam = act ? A.fcol_lam[line / A.n2] : 0;
+    C *src = A.data + (act ? line : 0) * N;
+    const TB *ph = A.dR + (size_t)lam * N;
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < N2; m++) v[m] = tmul(src[a + N1 * m], ph[a + N1 * m], false);
+    }
+    __syncthreads();  // tables
+    rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < N2; m++) src[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+    }
+  } else {
+    __shared__ long long s_off[LW];
+    __shared__ int s_lam[LW];
+    __shared__ int s_mixed[LW];
+    __shared__ int s_unit_line0[LW + 1];
+    const int ub = A.bundle_u0[blockIdx.x], ue = A.bundle_u0[blockIdx.x + 1];
+    const int nu = ue - ub;
+    const bool mix = (A.flags & FFT_MIX) != 0;
+    if (t == 0) {
+      int nl = 0;
+      for (int u = 0; u < nu; u++) {
+        s_unit_line0[u] = nl;
+        nl += A.units[ub + u].g;
+      }
+      s_unit_line0[nu] = nl;
+    }
+    __syncthreads();
+    const int nl = s_unit_line0[nu];
+    if (t < nl) {
+      int u = 0;
+      while (s_unit_line0[u + 1] <= t) u++;
+      const RUnit U = A.units[ub + u];
+      const int col = A.unit_cols[U.col0 + (t - s_unit_line0[u])];
+      s_off[t] = ((long long)col * A.n2 + U.r) * N;
+      s_lam[t] = A.fcol_lam[col];
+      s_mixed[t] = (mix && U.u0 >= 0) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int e = t; e < nl * N; e += NTH) {
+      const int l = e / N, p = e % N;
+      C x = A.data[s_off[l] + p];
+      if (!s_mixed[l]) x = tmul(x, A.dR[(size_t)s_lam[l] * N + p], false);
+      stg[l * SP + p] = x;
+    }
+    __syncthreads();
+    if (mix) {
+      for (int e = t; e < nu * N; e += NTH) {
+        const int u = e / N, p = e % N;
+        const RUnit U = A.units[ub + u];
+        if (U.u0 < 0) continue;
+        const int L0 = s_unit_line0[u];
+        C in[RP_GMAX];
+#pragma unroll
+        for (int K = 0; K < RP_GMAX; K++)
+          if (K < U.g) in[K] = stg[(L0 + K) * SP + p];
+#pragma unroll 1
+        for (int m = 0; m < U.g; m++) {
+          C acc;
+          acc.x = 0;
+          acc.y = 0;
+#pragma unroll
+          for (int K = 0; K < RP_GMAX; K++)
+            if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
+          stg[(L0 + m) * SP + p] = tmul(acc, A.dR[(size_t)s_lam[L0 + m] * N + p], false);
+        }
+      }
+      __syncthreads();
+    }
+    const bool act = L < nl;
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < N2; m++) v[m] = stg[L * SP + a + N1 * m];
+    }
+    rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = v[m];
+    }
+    __syncthreads();
+    if (mix) {
+      for (int e = t; e < nu * N; e += NTH) {
+        const int u = e / N, p = e % N;
+        const RUnit U = A.units[ub + u];
+        if (U.u0 < 0) continue;
+        const int L0 = s_unit_line0[u];
+        C in[RP_GMAX];
+#pragma unroll
+        for (int m = 0; m < RP_GMAX; m++)
+          if (m < U.g) in[m] = tmul(stg[(L0 + m) * SP + p], A.dR[(size_t)s_lam[L0 + m] * N + p], false);
+#pragma unroll 1
+        for (int K = 0; K < U.g; K++) {
+          C acc;
+          acc.x = 0;
+          acc.y = 0;
+#pragma unroll
+          for (int m = 0; m < RP_GMAX; m++)
+            if (m < U.g) acc = cadd(acc, cscale(in[m], A.umat[U.u0 + K * U.g + m]));
+          stg[(L0 + K) * SP + p] = acc;
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < nl * N; e += NTH) {
+      const int l = e / N, p = e % N;
+      C x = stg[l * SP + p];
+      if (!s_mixed[l]) x = tmul(x, A.dR[(size_t)s_lam[l] * N + p], false);
+      A.data[s_off[l] + p] = x;
+    }
+  }
+}
+
+template <typename T, int N1, int N2, bool MIXT, int LW>
+constexpr size_t rpass_rf_lds() {
+  constexpr int N = N1 * N2;
+  return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
+         ((size_t)LW * N1 * (N2 + 1) + (MIXT ? (size_t)LW * (N + N / 32 + 1) : 0)) * sizeof(typename cx_of<T>::type);
+}
+
+// r pass: lines (column, i) of N = n2 points at stride n1; 256 / N1
+// consecutive i of one column per work group (n1 a multiple of 256 / N1), so
+// the lanes of a load read consecutive addresses.
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
+  typedef typename cx_of<T>::type C;
+  typedef typename tab_of<T>::type TB;
+  constexpr int N = N1 * N2, LPB = 256 / N1, LS = LPB + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
+  TB *s_kin = s_twa + N1 * (N2 + 1);
+  C *xch = reinterpret_cast<C *>(s_kin + N);  // [N1][N2][LS]
+  const int t = threadIdx.x;
+  const int L = t % LPB, a = t / LPB;
+  rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
+  const int line = blockIdx.x * LPB + L;
+  const bool act = line < A.nlines;
+  const int lc = act ? line : 0;
+  const int i = lc % A.n1, c = lc / A.n1;
+  const int j = A.fcol_j[(blockIdx.x * LPB) / A.n1];
+  C *src = A.data + (size_t)c * A.n1 * N + i;
+  const size_t st = A.n1;
+  const TB *ph = A.dr + (size_t)j * N;
+  C v[N2];
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < N2; m++) v[m] = tmul(src[(size_t)(a + N1 * m) * st], ph[a + N1 * m], false);
+  }
+  __syncthreads();  // tables
+  auto xidx = [L](int n, int k) { return (n * N2 + k) * LS + L; };
+  rf_line<T, N1, N2>(v, a, act, xch, xidx, s_twa, s_kin);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < N2; m++) src[(size_t)(a + N1 * m) * st] = tmul(v[m], ph[a + N1 * m], false);
+  }
+}
+
+template <typename T, int N1, int N2>
+constexpr size_t rline_rf_lds() {
+  return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
+         (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type);
+}
+
+}  // namespace qd
