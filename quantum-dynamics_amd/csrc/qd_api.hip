@@ -192,6 +192,7 @@ struct qd_handle {
   // register-FFT passes (fused forward / kinetic / inverse with phases)
   bool rp_rf = false, rl_rf = false;
   int rp_lw = 0;
+  long long rp_slots = 1, rl_slots = 1;  // resident work groups of the persistent passes
   size_t tmx_lds = 0;
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
@@ -378,7 +379,8 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
     void *args[] = {&a};
     const int lw = h->rp_lw;
     const int nthreads = lw * rf_n1(h->n1);
-    const unsigned nblk = h->has_mix ? (unsigned)h->nbundle : (unsigned)(((long long)h->nfcol * h->n2 + lw - 1) / lw);
+    const unsigned nblk = h->has_mix ? (unsigned)h->nbundle
+                                     : (unsigned)std::min(h->rp_slots, ((long long)h->nfcol * h->n2 + lw - 1) / lw);
     QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(nthreads), args, lds, h->stream));
     QD_HIP(hipGetLastError());
     return QD_OK;
@@ -415,7 +417,7 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
     int lpb = 0;
     const void *fn = rline_rf_kernel<T>(h->n2, &lds, &lpb);
     void *args[] = {&a};
-    const unsigned nblk = (unsigned)(((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
+    const unsigned nblk = (unsigned)std::min(h->rl_slots, ((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
     QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, h->stream));
     QD_HIP(hipGetLastError());
     return QD_OK;
@@ -907,7 +909,8 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     size_t lds = 0;
     const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, h->has_mix, lw, &lds)
                                              : rpass_rf_kernel<float>(n1, h->has_mix, lw, &lds);
-    if (fn && lds <= 160 * 1024 && lw * n1f <= 1024) {
+    // without mixing a batch of lw lines must lie in one column
+    if (fn && lds <= 160 * 1024 && lw * n1f <= 1024 && (h->has_mix || n2 % lw == 0)) {
       h->rp_rf = true;
       h->rp_lw = lw;
     }
@@ -954,17 +957,25 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->rl_lds));
   }
   if (h->tmx_ks) QD_RC(setup_theta_mx(h));
+  int ncu = 0;
+  QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
   if (h->rp_rf) {
     size_t lds = 0;
     const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, h->has_mix, h->rp_lw, &lds)
                                              : rpass_rf_kernel<float>(n1, h->has_mix, h->rp_lw, &lds);
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nb = 0;
+    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, h->rp_lw * rf_n1(n1), lds));
+    h->rp_slots = std::max(1, nb) * std::max(1, ncu);
   }
   if (h->rl_rf) {
     size_t lds = 0;
     int lpb = 0;
     const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, &lds, &lpb) : rline_rf_kernel<float>(n2, &lds, &lpb);
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nb = 0;
+    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds));
+    h->rl_slots = std::max(1, nb) * std::max(1, ncu);
   }
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));

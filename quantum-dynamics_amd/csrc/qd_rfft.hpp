@@ -237,7 +237,7 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
   TB *s_twa = reinterpret_cast<TB *>(smem_raw);
   TB *s_kin = s_twa + N1 * XS;
   C *xch = reinterpret_cast<C *>(s_kin + N);  // [LW][LX]
-  C *stg = xch + LW * LX;                       // [LW][SP], mixing kernels only
+  C *stg = xch + LW * LX;                       // [LW][SP] (mixing) or the D_R rows
   const int t = threadIdx.x;
   const int a = t % N1, L = t / N1;
   rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
@@ -245,21 +245,49 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
   C *xb = xch + L * LX;
   auto xidx = [](int n, int k) { return n * XS + k; };
   if constexpr (!MIXT) {
+    // Persistent over batches of LW consecutive lines of one column (the host
+    // requires n2 % LW == 0), so the batch has one D_R row.  The next batch's
+    // points and D_R row are loaded while this one is transformed; the row
+    // goes through a double-buffered LDS copy.
+    constexpr int PHE = (N + NTH - 1) / NTH;
+    TB *s_ph = reinterpret_cast<TB *>(stg);  // [2][N]
     const long long nlines = (long long)A.nfcol * A.n2;
-    const long long line = (long long)blockIdx.x * LW + L;
-    const bool act = line < nlines;
-    const int lam = act ? A.fcol_lam[line / A.n2] : 0;
-    C *src = A.data + (act ? line : 0) * N;
-    const TB *ph = A.dR + (size_t)lam * N;
-    if (act) {
+    const long long nbatch = nlines / LW;
+    C raw[N2];
+    TB phr[PHE];
+    // The D_R row index of the batch to prefetch is read at the top of the
+    // previous iteration, when no other load is outstanding.
+    auto fetch = [&](long long b, int lam) {
+      const C *src = A.data + (b * LW + L) * N;
 #pragma unroll
-      for (int m = 0; m < N2; m++) v[m] = tmul(src[a + N1 * m], ph[a + N1 * m], false);
-    }
-    __syncthreads();  // tables
-    rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
-    if (act) {
+      for (int m = 0; m < N2; m++) raw[m] = src[a + N1 * m];
+      const TB *row = A.dR + (size_t)lam * N;
 #pragma unroll
-      for (int m = 0; m < N2; m++) src[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+      for (int q = 0; q < PHE; q++)
+        if (N % NTH == 0 || q * NTH + t < N) phr[q] = row[q * NTH + t];
+    };
+    auto lam_of = [&](long long b) { return A.fcol_lam[(b * LW) / A.n2]; };
+    long long b = blockIdx.x;
+    if (b < nbatch) fetch(b, lam_of(b));
+    int buf = 0;
+    for (; b < nbatch; b += gridDim.x) {
+      const long long bn = b + gridDim.x < nbatch ? b + gridDim.x : b;
+      const int lam_n = lam_of(bn);
+      TB *ph = s_ph + buf * N;
+#pragma unroll
+      for (int q = 0; q < PHE; q++)
+        if (N % NTH == 0 || q * NTH + t < N) ph[q * NTH + t] = phr[q];
+      __syncthreads();  // row (and, first time round, the tables)
+#pragma unroll
+      for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
+      // unconditional (the last batch fetches itself again) keeps the loop
+      // body straight-line, so the waits stay exact across iterations
+      fetch(bn, lam_n);
+      rf_line<T, N1, N2>(v, a, true, xb, xidx, s_twa, s_kin);
+      C *dst = A.data + (b * LW + L) * N;
+#pragma unroll
+      for (int m = 0; m < N2; m++) dst[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+      buf ^= 1;
     }
   } else {
     __shared__ long long s_off[LW];
@@ -366,12 +394,15 @@ template <typename T, int N1, int N2, bool MIXT, int LW>
 constexpr size_t rpass_rf_lds() {
   constexpr int N = N1 * N2;
   return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
-         ((size_t)LW * N1 * (N2 + 1) + (MIXT ? (size_t)LW * (N + N / 32 + 1) : 0)) * sizeof(typename cx_of<T>::type);
+         (size_t)LW * N1 * (N2 + 1) * sizeof(typename cx_of<T>::type) +
+         (MIXT ? (size_t)LW * (N + N / 32 + 1) * sizeof(typename cx_of<T>::type)
+               : (size_t)2 * N * sizeof(typename tab_of<T>::type));
 }
 
-// r pass: lines (column, i) of N = n2 points at stride n1; 256 / N1
-// consecutive i of one column per work group (n1 a multiple of 256 / N1), so
-// the lanes of a load read consecutive addresses.
+// r pass: lines (column, i) of N = n2 points at stride n1; a batch is 256 / N1
+// consecutive i of one column (n1 a multiple of 256 / N1), so the lanes of a
+// load read consecutive addresses.  Persistent over batches, with the next
+// batch loaded while the current one is transformed.
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   typedef typename cx_of<T>::type C;
@@ -384,32 +415,55 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   const int t = threadIdx.x;
   const int L = t % LPB, a = t / LPB;
   rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
-  const int line = blockIdx.x * LPB + L;
-  const bool act = line < A.nlines;
-  const int lc = act ? line : 0;
-  const int i = lc % A.n1, c = lc / A.n1;
-  const int j = A.fcol_j[(blockIdx.x * LPB) / A.n1];
-  C *src = A.data + (size_t)c * A.n1 * N + i;
   const size_t st = A.n1;
-  const TB *ph = A.dr + (size_t)j * N;
-  C v[N2];
-  if (act) {
+  // persistent: batch b = LPB consecutive i of one column (n1 % LPB == 0);
+  // the next batch's points and D_r row are loaded while this one is
+  // transformed, the row through a double-buffered LDS copy
+  constexpr int PHE = (N + 255) / 256;
+  TB *s_ph = reinterpret_cast<TB *>(xch + N1 * N2 * LS);  // [2][N]
+  const int nbatch = A.nlines / LPB;
+  C raw[N2], v[N2];
+  TB phr[PHE];
+  auto fetch = [&](int b, int jj) {
+    const int line = b * LPB + L;
+    const C *src = A.data + (size_t)(line / A.n1) * A.n1 * N + line % A.n1;
 #pragma unroll
-    for (int m = 0; m < N2; m++) v[m] = tmul(src[(size_t)(a + N1 * m) * st], ph[a + N1 * m], false);
-  }
-  __syncthreads();  // tables
+    for (int m = 0; m < N2; m++) raw[m] = src[(size_t)(a + N1 * m) * st];
+    const TB *row = A.dr + (size_t)jj * N;
+#pragma unroll
+    for (int q = 0; q < PHE; q++)
+      if (N % 256 == 0 || q * 256 + t < N) phr[q] = row[q * 256 + t];
+  };
+  auto j_of = [&](int b) { return A.fcol_j[(b * LPB) / A.n1]; };
+  int b = blockIdx.x;
+  if (b < nbatch) fetch(b, j_of(b));
   auto xidx = [L](int n, int k) { return (n * N2 + k) * LS + L; };
-  rf_line<T, N1, N2>(v, a, act, xch, xidx, s_twa, s_kin);
-  if (act) {
+  int buf = 0;
+  for (; b < nbatch; b += gridDim.x) {
+    const int bn = b + (int)gridDim.x < nbatch ? b + (int)gridDim.x : b;
+    const int j_n = j_of(bn);  // read while nothing else is outstanding
+    TB *ph = s_ph + buf * N;
 #pragma unroll
-    for (int m = 0; m < N2; m++) src[(size_t)(a + N1 * m) * st] = tmul(v[m], ph[a + N1 * m], false);
+    for (int q = 0; q < PHE; q++)
+      if (N % 256 == 0 || q * 256 + t < N) ph[q * 256 + t] = phr[q];
+    __syncthreads();  // row (and, first time round, the tables)
+    const int line = b * LPB + L;
+#pragma unroll
+    for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
+    fetch(bn, j_n);
+    rf_line<T, N1, N2>(v, a, true, xch, xidx, s_twa, s_kin);
+    C *dst = A.data + (size_t)(line / A.n1) * A.n1 * N + line % A.n1;
+#pragma unroll
+    for (int m = 0; m < N2; m++) dst[(size_t)(a + N1 * m) * st] = tmul(v[m], ph[a + N1 * m], false);
+    buf ^= 1;
   }
 }
 
 template <typename T, int N1, int N2>
 constexpr size_t rline_rf_lds() {
   return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
-         (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type);
+         (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type) +
+         (size_t)2 * N1 * N2 * sizeof(typename tab_of<T>::type);
 }
 
 }  // namespace qd
