@@ -965,6 +965,17 @@ __device__ __forceinline__ float2 buf_ld_f2(__amdgpu_buffer_rsrc_t r, unsigned o
   const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
   return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// 8- or 16-byte element load through a descriptor (float2, double2, float4)
+template <typename V>
+__device__ __forceinline__ V buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  static_assert(sizeof(V) == 8 || sizeof(V) == 16, "8- or 16-byte elements");
+  if constexpr (sizeof(V) == 8) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  } else {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+  }
+}
 __device__ __forceinline__ void buf_st_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
   u32x2 v;
   v.x = __float_as_uint(x.x);

@@ -257,21 +257,27 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
     TB phr[PHE];
     // The D_R row index of the batch to prefetch is read at the top of the
     // previous iteration, when no other load is outstanding.
-    auto fetch = [&](long long b, int lam) {
-      const C *src = A.data + (b * LW + L) * N;
+    // Loads go through descriptors over the batch and its row; with `has`
+    // false the descriptors are empty, the loads return zeros and move no
+    // memory (the loop body stays branch-free).
+    auto fetch = [&](long long b, int lam, bool has) {
+      const __amdgpu_buffer_rsrc_t rd = buf_rsrc(A.data + b * LW * N, has ? (unsigned)(LW * N * sizeof(C)) : 0u);
+      const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.dR + (size_t)lam * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
 #pragma unroll
-      for (int m = 0; m < N2; m++) raw[m] = src[a + N1 * m];
-      const TB *row = A.dR + (size_t)lam * N;
+      for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)((L * N + a + N1 * m) * sizeof(C)));
 #pragma unroll
       for (int q = 0; q < PHE; q++)
-        if (N % NTH == 0 || q * NTH + t < N) phr[q] = row[q * NTH + t];
+        if (N % NTH == 0 || q * NTH + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * NTH + t) * sizeof(TB)));
     };
-    auto lam_of = [&](long long b) { return A.fcol_lam[(b * LW) / A.n2]; };
+    // one column per batch: the index is uniform (readfirstlane keeps the
+    // descriptors in scalar registers)
+    auto lam_of = [&](long long b) { return __builtin_amdgcn_readfirstlane(A.fcol_lam[(b * LW) / A.n2]); };
     long long b = blockIdx.x;
-    if (b < nbatch) fetch(b, lam_of(b));
+    if (b < nbatch) fetch(b, lam_of(b), true);
     int buf = 0;
     for (; b < nbatch; b += gridDim.x) {
-      const long long bn = b + gridDim.x < nbatch ? b + gridDim.x : b;
+      const bool has_n = b + gridDim.x < nbatch;
+      const long long bn = has_n ? b + gridDim.x : b;
       const int lam_n = lam_of(bn);
       TB *ph = s_ph + buf * N;
 #pragma unroll
@@ -280,9 +286,9 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
       __syncthreads();  // row (and, first time round, the tables)
 #pragma unroll
       for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
-      // unconditional (the last batch fetches itself again) keeps the loop
+      // unconditional (empty descriptors past the last batch) keeps the loop
       // body straight-line, so the waits stay exact across iterations
-      fetch(bn, lam_n);
+      fetch(bn, lam_n, has_n);
       rf_line<T, N1, N2>(v, a, true, xb, xidx, s_twa, s_kin);
       C *dst = A.data + (b * LW + L) * N;
 #pragma unroll
@@ -424,23 +430,28 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   const int nbatch = A.nlines / LPB;
   C raw[N2], v[N2];
   TB phr[PHE];
-  auto fetch = [&](int b, int jj) {
-    const int line = b * LPB + L;
-    const C *src = A.data + (size_t)(line / A.n1) * A.n1 * N + line % A.n1;
+  // descriptors over the batch's column slab and its row; empty when `has`
+  // is false (loads return zeros, no memory traffic)
+  auto fetch = [&](int b, int jj, bool has) {
+    const int l0 = b * LPB;
+    const C *base = A.data + (size_t)(l0 / A.n1) * A.n1 * N + l0 % A.n1;
+    const __amdgpu_buffer_rsrc_t rd =
+        buf_rsrc(base, has ? (unsigned)(((size_t)(N - 1) * A.n1 + LPB) * sizeof(C)) : 0u);
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.dr + (size_t)jj * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
 #pragma unroll
-    for (int m = 0; m < N2; m++) raw[m] = src[(size_t)(a + N1 * m) * st];
-    const TB *row = A.dr + (size_t)jj * N;
+    for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)(((size_t)(a + N1 * m) * st + L) * sizeof(C)));
 #pragma unroll
     for (int q = 0; q < PHE; q++)
-      if (N % 256 == 0 || q * 256 + t < N) phr[q] = row[q * 256 + t];
+      if (N % 256 == 0 || q * 256 + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * 256 + t) * sizeof(TB)));
   };
-  auto j_of = [&](int b) { return A.fcol_j[(b * LPB) / A.n1]; };
+  auto j_of = [&](int b) { return __builtin_amdgcn_readfirstlane(A.fcol_j[(b * LPB) / A.n1]); };
   int b = blockIdx.x;
-  if (b < nbatch) fetch(b, j_of(b));
+  if (b < nbatch) fetch(b, j_of(b), true);
   auto xidx = [L](int n, int k) { return (n * N2 + k) * LS + L; };
   int buf = 0;
   for (; b < nbatch; b += gridDim.x) {
-    const int bn = b + (int)gridDim.x < nbatch ? b + (int)gridDim.x : b;
+    const bool has_n = b + (int)gridDim.x < nbatch;
+    const int bn = has_n ? b + (int)gridDim.x : b;
     const int j_n = j_of(bn);  // read while nothing else is outstanding
     TB *ph = s_ph + buf * N;
 #pragma unroll
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
     const int line = b * LPB + L;
 #pragma unroll
     for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
-    fetch(bn, j_n);
+    fetch(bn, j_n, has_n);
     rf_line<T, N1, N2>(v, a, true, xch, xidx, s_twa, s_kin);
     C *dst = A.data + (size_t)(line / A.n1) * A.n1 * N + line % A.n1;
 #pragma unroll
