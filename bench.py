@@ -1,14 +1,26 @@
 """Throughput of the wavepacket propagator on one MI355X.
 
-Metric: grid-point-steps/s = sum over blocks of n1*n2*ntheta x steps / seconds,
+Metric: grid-point-steps/s = sum over blocks of n1*n2*n_theta x steps / seconds,
 with psi, V and all tables resident in HBM when the timed region starts.
 
-`value` is the batched mixed-precision path (FP32 psi/FFT/phases, BF16 split
-theta GEMM with FP32 accumulation, FP64 flux accumulation) on the F+H2 J=0
-256x192x120 grid.  The plain FP64 path (one kernel per reference operator)
-and the fused FP64 path are timed on the same grid and reported beside it.
+`value` is the batched mixed-precision step (FP32 psi/FFT/phases, BF16 split
+theta GEMM with FP32 accumulation, FP64 flux accumulation) on the batched
+F+H2 workload: J = 0..20, every K = 0..min(J, 60), j_max = 60 -- 231
+Coriolis-coupled (J, K) blocks of 256 x 192 x (61 - K) points, 616.9 M
+grid points, advanced together every step.
+
+Beside it, on the same grids:
+  plain_fp64   the reference's operator sequence in FP64, one kernel per
+               operator (V phase, theta-fwd, rotation, FFTs, kinetic, inverse
+               FFTs, rotation, theta-bwd, dump), run block by block: the time
+               per step is the sum over the 231 blocks of the measured step
+               time of a 256 x 192 x (61 - K) single-block propagation
+               (no Coriolis coupling, which only favours it);
+  fused_fp64   the batched step in FP64;
+  single_grid  the F+H2 J = 0 256 x 192 x 120 grid: mixed, fused FP64, plain FP64.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--workload batched|single] [--calibrate]
 """
 from __future__ import annotations
 
@@ -19,35 +31,70 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per grid point and step of each kernel class (M = ntheta):
-# theta-fwd reads psi and V and writes the FBR array; the R and r passes and
-# theta-bwd each read and write one complex array.
+# algorithmic bytes per grid point of one launch of each kernel class
+# (68 B per grid-point-step mixed, 136 B FP64):
+# theta-fwd reads psi (s_c) and V (s_v) and writes the FBR array; the R and r
+# passes and theta-bwd each read and write one complex array.
 B_ALG = {
-    "mixed": {"theta": 8 + 4 + 8 + 8 + 8, "theta_fwd": 8 + 4 + 8, "fft_R": 16, "fft_r": 16, "total": 68},
-    "fp64": {"theta": 16 + 8 + 16 + 16 + 16, "theta_fwd": 16 + 8 + 16, "fft_R": 32, "fft_r": 32, "total": 136},
+    "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "fft_R": 16, "fft_r": 16, "total": 68},
+    "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
+}
+# HBM bytes per launch and grid point, measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/hbm_traffic_*.md
+MEASURED_B_PER_POINT = {
+    "theta": (24.2 + 16.2) / 2,   # mean of theta-fwd and theta-bwd (the event class averages both)
+    "fft_R": 16.7,
+    "fft_r": 16.5,
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
+KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER))
 
 
-def timed_run(qd, prob, precision, pipeline, steps, warmup, profile=0):
-    p = prob.copy()
-    with qd.Propagator(p, precision=precision, pipeline=pipeline, profile=profile) as prop:
-        if warmup:
-            prop.propagate(warmup)
-        t0 = time.perf_counter()
-        prop.propagate(steps)
-        wall = time.perf_counter() - t0
-        step_ms = prop.timing(0)
-        kern = {k: prop.timing(i) for k, i in
-                (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX),
-                 ("other", TK_OTHER))}
-    return wall, step_ms, kern
+def _run(prop, steps, warmup, profile):
+    if warmup:
+        prop.propagate(warmup)
+    t0 = time.perf_counter()
+    prop.propagate(steps)          # returns after the device has finished
+    wall = time.perf_counter() - t0
+    out = {"wall_ms_per_step": 1e3 * wall / steps, "event_ms_per_step": prop.timing(0)}
+    if profile:
+        out["kernel_ms"] = {k: prop.timing(i) for k, i in KCLASSES}
+    return out
+
+
+def timed_single(qd, prob, precision, pipeline, steps, warmup, profile=0):
+    with qd.Propagator(prob.copy(), precision=precision, pipeline=pipeline, profile=profile) as prop:
+        return _run(prop, steps, warmup, profile)
+
+
+def timed_batched(qd, jk, precision, steps, warmup, profile=0):
+    with qd.propagator.JKPropagator(jk, precision=precision, profile=profile) as prop:
+        return _run(prop, steps, warmup, profile)
+
+
+def batched_problem(qd, Jmax=20, jmax=60):
+    pb = qd.problem
+    base = pb.config(2, ntheta=jmax + 1)
+    return pb.make_jk_problem(base, pb.F_H2, list(range(Jmax + 1)), jmax=jmax)
+
+
+def plain_fp64_same_grids(qd, jk, steps, warmup):
+    """Plain FP64 step time summed over the blocks of `jk` (one single-block
+    run per distinct theta size)."""
+    pb = qd.problem
+    count = {}
+    for b in jk.blocks:
+        count[b.n] = count.get(b.n, 0) + 1
+    per_n = {}
+    for n in sorted(count):
+        prob = pb.config(2, ntheta=n)
+        per_n[n] = timed_single(qd, prob, "fp64", "plain", steps, warmup)["wall_ms_per_step"]
+    ms = sum(count[n] * per_n[n] for n in count)
+    return ms, per_n, count
 
 
 def cpu_baseline(prob, seconds_target=15.0):
@@ -59,17 +106,17 @@ def cpu_baseline(prob, seconds_target=15.0):
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = min(cores, 16)
-    # one step to size the sample
     t0 = time.perf_counter()
-    oracle.time_evolution(prob, total_steps=1, crp_every=1, diagnostics=1, nthreads=cores)
-    t1 = time.perf_counter() - t0
-    n = int(max(2, min(200, seconds_target / max(t1, 1e-3))))
+    oracle.time_evolution(prob, total_steps=2, crp_every=1, diagnostics=1, nthreads=cores)
+    t1 = (time.perf_counter() - t0) / 2
+    n = int(max(2, min(400, seconds_target / max(t1, 1e-3))))
     t0 = time.perf_counter()
     oracle.time_evolution(prob, total_steps=n, crp_every=1, diagnostics=1, nthreads=cores)
     dt = time.perf_counter() - t0
     return {"value": prob.grid_points * n / dt, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
-            "sample": "%d steps of the same grid (P(E) and energies every step), C++/OpenMP oracle, %.1f s"
-                      % (n, dt)}
+            "sample": "%d steps of the %dx%dx%d grid (F+H2 J=0, P(E) and energies every step), C++/OpenMP "
+                      "oracle on %d threads, %.1f s; the batched workload's blocks have the same per-point cost"
+                      % (n, prob.n1, prob.n2, prob.ntheta, cores, dt)}
 
 
 def calibrate_copy(mib=1024, reps=5):
@@ -88,13 +135,25 @@ def calibrate_copy(mib=1024, reps=5):
     print(json.dumps({"calibration_copy_MiB": mib, "reps": reps}))
 
 
+def roofline(kern, npt, prec="mixed"):
+    """Dominant kernel class of the step, its algorithmic bytes per launch
+    over its event-timed launch duration."""
+    dom = max(("theta", "fft_R", "fft_r"), key=lambda k: kern[k])
+    b = B_ALG[prec]
+    per_point = {"theta": (b["theta_fwd"] + b["theta_bwd"]) / 2.0, "fft_R": b["fft_R"], "fft_r": b["fft_r"]}[dom]
+    achieved = per_point * npt / (kern[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[dom] * npt,
+            "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--workload", choices=("batched", "single"), default="batched")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a device-to-device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -102,52 +161,80 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the propagator on a single MI355X; "
-              "J blocks would shard across GPUs by J (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the propagator runs on a single MI355X here; the batched "
+              "workload would shard by J across GPUs (DESIGN.md section 6)", file=sys.stderr)
         sys.exit(2)
     qd = importlib.import_module("quantum-dynamics_amd")
-    prob = qd.problem.config(args.workload)
-    npt = prob.grid_points
+    K, W = args.steps, args.warmup
 
-    wall_m, step_m, _ = timed_run(qd, prob, "mixed", "fused", args.steps, args.warmup)
-    _, _, kern_m = timed_run(qd, prob, "mixed", "fused", max(20, args.steps // 4), 5, profile=1)
-    wall_p, step_p, _ = timed_run(qd, prob, "fp64", "plain", max(20, args.steps // 2), 5)
-    wall_f, step_f, _ = timed_run(qd, prob, "fp64", "fused", max(20, args.steps // 2), 5)
-    _, _, kern_f = timed_run(qd, prob, "fp64", "fused", max(20, args.steps // 4), 5, profile=1)
+    # single grid (F+H2 J=0, 256 x 192 x 120)
+    prob = qd.problem.config(2)
+    npt1 = prob.grid_points
+    s_mixed = timed_single(qd, prob, "mixed", "fused", K, W)
+    s_mixed.update(timed_single(qd, prob, "mixed", "fused", max(20, K // 4), 5, profile=1))
+    s_f64 = timed_single(qd, prob, "fp64", "fused", max(20, K // 2), 5, profile=1)
+    s_plain = timed_single(qd, prob, "fp64", "plain", max(20, K // 2), 5)
+    single = {
+        "workload": "F+H2 J=0 256x192x120", "grid_points": npt1,
+        "mixed": {"value": npt1 / (s_mixed["wall_ms_per_step"] * 1e-3), **s_mixed},
+        "fused_fp64": {"value": npt1 / (s_f64["wall_ms_per_step"] * 1e-3), **s_f64},
+        "plain_fp64": {"value": npt1 / (s_plain["wall_ms_per_step"] * 1e-3), **s_plain},
+    }
+    single["speedup_mixed_vs_plain_fp64"] = single["mixed"]["value"] / single["plain_fp64"]["value"]
+    single["roofline"] = roofline(s_mixed["kernel_ms"], npt1)
 
-    value = npt / (step_m * 1e-3)
-    plain = npt / (step_p * 1e-3)
-    fused64 = npt / (step_f * 1e-3)
-    # dominant kernel class of the mixed step
-    dom = max(("theta", "fft_R", "fft_r"), key=lambda k: kern_m[k])
-    per_launch_bytes = {
-        "theta": B_ALG["mixed"]["theta"] / 2.0,  # two theta launches per step
-        "fft_R": B_ALG["mixed"]["fft_R"],
-        "fft_r": B_ALG["mixed"]["fft_r"],
-    }[dom] * npt
-    achieved = per_launch_bytes / (kern_m[dom] * 1e-3) / 1e9
+    if args.workload == "batched":
+        jk = batched_problem(qd)
+        npt = jk.grid_points
+        label = ("F+H2 J=0..20, K=0..min(J,60), j_max=60: 231 Coriolis-coupled (J,K) blocks of 256x192x(61-K), "
+                 "mixed precision")
+        m = timed_batched(qd, jk, "mixed", K, W)
+        prof = timed_batched(qd, jk, "mixed", max(10, K // 10), 2, profile=1)
+        f64 = timed_batched(qd, jk, "fp64", max(10, K // 10), 2)
+        plain_ms, plain_per_n, counts = plain_fp64_same_grids(qd, jk, max(10, K // 10), 3)
+        step_ms = m["wall_ms_per_step"]
+        value = npt / (step_ms * 1e-3)
+        plain = {"value": npt / (plain_ms * 1e-3), "ms_per_step": plain_ms,
+                 "how": "sum over blocks of single-block plain FP64 step times (per theta size, ms)",
+                 "per_theta_size_ms": plain_per_n, "blocks_per_theta_size": counts}
+        fused = {"value": npt / (f64["wall_ms_per_step"] * 1e-3), "ms_per_step": f64["wall_ms_per_step"]}
+        kern = prof["kernel_ms"]
+        rf = roofline(kern, npt)
+        cfg = {"workload": label, "blocks": len(jk.blocks), "n1": jk.base.n1, "n2": jk.base.n2,
+               "theta_sizes": "41..61", "grid_points": npt, "dt": jk.base.dt}
+    else:
+        npt = npt1
+        step_ms = s_mixed["wall_ms_per_step"]
+        value = single["mixed"]["value"]
+        plain = {"value": single["plain_fp64"]["value"], "ms_per_step": s_plain["wall_ms_per_step"]}
+        fused = {"value": single["fused_fp64"]["value"], "ms_per_step": s_f64["wall_ms_per_step"]}
+        kern = s_mixed["kernel_ms"]
+        rf = single["roofline"]
+        m = s_mixed
+        cfg = {"workload": "F+H2 J=0 256x192x120 mixed precision", "n1": prob.n1, "n2": prob.n2,
+               "ntheta": prob.ntheta, "grid_points": npt, "dt": prob.dt}
+
     out = {
         "metric": "grid-point-steps/s (plain FP64 vs batched mixed) at |dP(E)|<=1e-4",
         "value": value,
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": step_m,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": step_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (bf16x3 split MFMA theta GEMM, f64 flux)",
-        "data": "synthetic (LEPS surface, analytic packet)",
-        "config": {"workload": prob.name + " mixed precision", "n1": prob.n1, "n2": prob.n2,
-                   "ntheta": prob.ntheta, "grid_points": npt, "dt": prob.dt},
-        "plain_fp64": {"value": plain, "ms_per_step": step_p},
-        "fused_fp64": {"value": fused64, "ms_per_step": step_f},
-        "speedup_vs_plain_fp64": value / plain,
-        "kernel_ms": {"mixed": kern_m, "fp64_fused": kern_f},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "step_achieved_GBs": B_ALG["mixed"]["total"] * value / 1e9},
+        "data": "synthetic (analytic LEPS surface on the Jacobi grid, analytic initial packet)",
+        "config": cfg,
+        "event_ms_per_step": m["event_ms_per_step"],
+        "plain_fp64": plain,
+        "fused_fp64": fused,
+        "speedup_vs_plain_fp64": value / plain["value"],
+        "kernel_ms": kern,
+        "roofline": rf,
+        "single_grid": single,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

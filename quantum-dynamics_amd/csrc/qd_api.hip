@@ -910,7 +910,9 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, h->has_mix, lw, &lds)
                                              : rpass_rf_kernel<float>(n1, h->has_mix, lw, &lds);
     // without mixing a batch of lw lines must lie in one column
-    if (fn && lds <= 160 * 1024 && lw * n1f <= 1024 && (h->has_mix || n2 % lw == 0)) {
+    // the mixing kernel also holds 2 x lw x 40 unit-matrix entries statically
+    const size_t lds_static = h->has_mix ? 2 * (size_t)lw * 40 * (h->prec == QD_PREC_FP64 ? 8 : 4) + 1024 : 0;
+    if (fn && lds + lds_static <= 160 * 1024 && lw * n1f <= 1024 && (h->has_mix || n2 % lw == 0)) {
       h->rp_rf = true;
       h->rp_lw = lw;
     }

@@ -17,6 +17,7 @@
 // step, the norm).
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "qd_kernels.hpp"
@@ -223,6 +224,65 @@ __device__ __forceinline__ void rf_tables(typename tab_of<T>::type *twa, typenam
   }
 }
 
+// out[m] = sum_K M[K][m] in[K] over the g (<= G) lines of a mixing unit at
+// one point p, in place in the staged lines.  M is G x G row-major in LDS,
+// zero-padded beyond g; rows are read as broadcast 4-wide vectors.
+template <typename T, int G, int SP>
+__device__ __forceinline__ void rf_mix_point(typename cx_of<T>::type *stg, int L0, int g, int p, const T *M) {
+  typedef typename cx_of<T>::type C;
+  typedef typename std::conditional<sizeof(T) == 8, double4, float4>::type V4;
+  C acc[G];
+#pragma unroll
+  for (int m = 0; m < G; m++) {
+    acc[m].x = 0;
+    acc[m].y = 0;
+  }
+#pragma unroll
+  for (int K = 0; K < G; K++) {
+    if (K < g) {
+      const C x = stg[(L0 + K) * SP + p];
+#pragma unroll
+      for (int m4 = 0; m4 < G / 4; m4++) {
+        const V4 u = *reinterpret_cast<const V4 *>(M + K * G + 4 * m4);
+        acc[4 * m4 + 0].x += x.x * u.x;
+        acc[4 * m4 + 0].y += x.y * u.x;
+        acc[4 * m4 + 1].x += x.x * u.y;
+        acc[4 * m4 + 1].y += x.y * u.y;
+        acc[4 * m4 + 2].x += x.x * u.z;
+        acc[4 * m4 + 2].y += x.y * u.z;
+        acc[4 * m4 + 3].x += x.x * u.w;
+        acc[4 * m4 + 3].y += x.y * u.w;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < G; m++)
+    if (m < g) stg[(L0 + m) * SP + p] = acc[m];
+}
+
+// Mixing of every mixed unit of a bundle, one (unit, point) per thread.
+template <typename T, int N, int SP, int NTH>
+__device__ __forceinline__ void rf_mix_bundle(typename cx_of<T>::type *stg, int nu, const int *ug, const int *line0,
+                                              const int *uoff, const T *mats) {
+  for (int e = threadIdx.x; e < nu * N; e += NTH) {
+    const int u = e / N, p = e % N;
+    const int g = ug[u];
+    if (g == 0) continue;
+    const int L0 = line0[u];
+    const T *M = mats + uoff[u];
+    switch ((g + 3) >> 2) {
+      case 1: rf_mix_point<T, 4, SP>(stg, L0, g, p, M); break;
+      case 2: rf_mix_point<T, 8, SP>(stg, L0, g, p, M); break;
+      case 3: rf_mix_point<T, 12, SP>(stg, L0, g, p, M); break;
+      case 4: rf_mix_point<T, 16, SP>(stg, L0, g, p, M); break;
+      case 5: rf_mix_point<T, 20, SP>(stg, L0, g, p, M); break;
+      case 6: rf_mix_point<T, 24, SP>(stg, L0, g, p, M); break;
+      case 7: rf_mix_point<T, 28, SP>(stg, L0, g, p, M); break;
+      default: rf_mix_point<T, 32, SP>(stg, L0, g, p, M); break;
+    }
+  }
+}
+
 // R pass: lines (column, r) of N = n1 contiguous points; LW lines per work
 // group of LW * N1 threads (LW >= the largest mixing group).  Without K mixing each thread loads its points straight from
 // HBM; with mixing the lines of a bundle are staged in LDS, mixed into the l
@@ -296,20 +356,34 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
       buf ^= 1;
     }
   } else {
+    // A bundle of whole mixing units (the K blocks of one (J, j, r)) is
+    // staged in LDS:  load -> K-to-l mixing -> D_R/2, FFT, kinetic, inverse
+    // FFT, D_R/2 (register stages; the phase rows are read with the points)
+    // -> l-to-K mixing -> store.  The unit matrices and descriptors are
+    // copied to LDS first, so the mixing loops touch no global memory.
     __shared__ long long s_off[LW];
     __shared__ int s_lam[LW];
-    __shared__ int s_mixed[LW];
     __shared__ int s_unit_line0[LW + 1];
+    __shared__ int s_ug[LW], s_uoff[LW + 1];
+    // U (row K, column m) and its transpose per mixed unit, padded to G4 = 4 ceil(g/4)
+    __shared__ __align__(32) T s_um[LW * 40], s_umt[LW * 40];
     const int ub = A.bundle_u0[blockIdx.x], ue = A.bundle_u0[blockIdx.x + 1];
     const int nu = ue - ub;
     const bool mix = (A.flags & FFT_MIX) != 0;
     if (t == 0) {
-      int nl = 0;
+      int nl = 0, no = 0;
       for (int u = 0; u < nu; u++) {
+        const RUnit U = A.units[ub + u];
         s_unit_line0[u] = nl;
-        nl += A.units[ub + u].g;
+        nl += U.g;
+        const bool mixed = mix && U.u0 >= 0;
+        s_ug[u] = mixed ? U.g : 0;
+        s_uoff[u] = no;
+        const int g4 = (U.g + 3) & ~3;
+        no += mixed ? g4 * g4 : 0;
       }
       s_unit_line0[nu] = nl;
+      s_uoff[nu] = no;
     }
     __syncthreads();
     const int nl = s_unit_line0[nu];
@@ -320,78 +394,62 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
       const int col = A.unit_cols[U.col0 + (t - s_unit_line0[u])];
       s_off[t] = ((long long)col * A.n2 + U.r) * N;
       s_lam[t] = A.fcol_lam[col];
-      s_mixed[t] = (mix && U.u0 >= 0) ? 1 : 0;
     }
-    __syncthreads();
-    for (int e = t; e < nl * N; e += NTH) {
-      const int l = e / N, p = e % N;
-      C x = A.data[s_off[l] + p];
-      if (!s_mixed[l]) x = tmul(x, A.dR[(size_t)s_lam[l] * N + p], false);
-      stg[l * SP + p] = x;
-    }
-    __syncthreads();
-    if (mix) {
-      for (int e = t; e < nu * N; e += NTH) {
-        const int u = e / N, p = e % N;
-        const RUnit U = A.units[ub + u];
-        if (U.u0 < 0) continue;
-        const int L0 = s_unit_line0[u];
-        C in[RP_GMAX];
-#pragma unroll
-        for (int K = 0; K < RP_GMAX; K++)
-          if (K < U.g) in[K] = stg[(L0 + K) * SP + p];
-#pragma unroll 1
-        for (int m = 0; m < U.g; m++) {
-          C acc;
-          acc.x = 0;
-          acc.y = 0;
-#pragma unroll
-          for (int K = 0; K < RP_GMAX; K++)
-            if (K < U.g) acc = cadd(acc, cscale(in[K], A.umat[U.u0 + K * U.g + m]));
-          stg[(L0 + m) * SP + p] = tmul(acc, A.dR[(size_t)s_lam[L0 + m] * N + p], false);
-        }
+    for (int u = 0; u < nu; u++) {
+      const int g = s_ug[u];
+      if (g == 0) continue;
+      const int g4 = (g + 3) & ~3;
+      const T *src = A.umat + A.units[ub + u].u0;
+      for (int i = t; i < g4 * g4; i += NTH) {
+        const int r = i / g4, c = i % g4;
+        const bool in = r < g && c < g;
+        s_um[s_uoff[u] + i] = in ? src[r * g + c] : (T)0;
+        s_umt[s_uoff[u] + i] = in ? src[c * g + r] : (T)0;
       }
+    }
+    __syncthreads();
+    // fixed trip count: all of a thread's loads are in flight together
+    constexpr int PER = (LW * N + NTH - 1) / NTH;
+    {
+      C x[PER];
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int e = t + i * NTH;
+        if (e < nl * N) x[i] = A.data[s_off[e / N] + e % N];
+      }
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int e = t + i * NTH;
+        if (e < nl * N) stg[(e / N) * SP + e % N] = x[i];
+      }
+    }
+    __syncthreads();
+    // K -> l: c_m = sum_K U[K, m] c_K
+    if (mix) {
+      rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_ug, s_unit_line0, s_uoff, s_um);
       __syncthreads();
     }
     const bool act = L < nl;
+    const TB *ph = A.dR + (size_t)(act ? s_lam[L] : 0) * N;
     if (act) {
 #pragma unroll
-      for (int m = 0; m < N2; m++) v[m] = stg[L * SP + a + N1 * m];
+      for (int m = 0; m < N2; m++) v[m] = tmul(stg[L * SP + a + N1 * m], ph[a + N1 * m], false);
     }
     rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
     if (act) {
 #pragma unroll
-      for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = v[m];
+      for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
     }
     __syncthreads();
+    // l -> K: c_K = sum_m U[K, m] c_m
     if (mix) {
-      for (int e = t; e < nu * N; e += NTH) {
-        const int u = e / N, p = e % N;
-        const RUnit U = A.units[ub + u];
-        if (U.u0 < 0) continue;
-        const int L0 = s_unit_line0[u];
-        C in[RP_GMAX];
-#pragma unroll
-        for (int m = 0; m < RP_GMAX; m++)
-          if (m < U.g) in[m] = tmul(stg[(L0 + m) * SP + p], A.dR[(size_t)s_lam[L0 + m] * N + p], false);
-#pragma unroll 1
-        for (int K = 0; K < U.g; K++) {
-          C acc;
-          acc.x = 0;
-          acc.y = 0;
-#pragma unroll
-          for (int m = 0; m < RP_GMAX; m++)
-            if (m < U.g) acc = cadd(acc, cscale(in[m], A.umat[U.u0 + K * U.g + m]));
-          stg[(L0 + K) * SP + p] = acc;
-        }
-      }
+      rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_ug, s_unit_line0, s_uoff, s_umt);
       __syncthreads();
     }
-    for (int e = t; e < nl * N; e += NTH) {
-      const int l = e / N, p = e % N;
-      C x = stg[l * SP + p];
-      if (!s_mixed[l]) x = tmul(x, A.dR[(size_t)s_lam[l] * N + p], false);
-      A.data[s_off[l] + p] = x;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = t + i * NTH;
+      if (e < nl * N) A.data[s_off[e / N] + e % N] = stg[(e / N) * SP + e % N];
     }
   }
 }

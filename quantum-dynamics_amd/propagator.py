@@ -257,8 +257,10 @@ class JKPropagator(Propagator):
         self._psi_blocks = []
         blocks = (_abi.qd_jk_block * len(jk.blocks))()
         n12 = base.n1 * base.n2
+        psi = np.ascontiguousarray(jk.psi)
         for i, b in enumerate(jk.blocks):
-            z = np.ascontiguousarray(jk.psi[b.offset:b.offset + n12 * b.n]).view(np.float64).copy()
+            # a view (the library copies psi to the device while creating the handle)
+            z = psi[b.offset:b.offset + n12 * b.n].view(np.float64)
             self._psi_blocks.append(z)
             blocks[i].J = b.J
             blocks[i].K = b.K
