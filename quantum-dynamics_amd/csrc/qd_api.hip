@@ -193,6 +193,8 @@ struct qd_handle {
   bool rp_rf = false, rl_rf = false;
   int rp_lw = 0;
   long long rp_slots = 1, rl_slots = 1;  // resident work groups of the persistent passes
+  int *bundle_l0 = nullptr, *line_lam = nullptr;  // mixing R pass: per-bundle line tables
+  long long *line_off = nullptr;
   size_t tmx_lds = 0;
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
@@ -373,14 +375,18 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   if (h->has_mix && (flags & FFT_PHASE)) a.flags |= FFT_MIX;
   a.plan = h->planR;
   a.nfcol = (int)h->nfcol;
+  a.bundle_l0 = h->bundle_l0;
+  a.line_off = h->line_off;
+  a.line_lam = h->line_lam;
+  a.nbundle = h->nbundle;
   if (h->rp_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
     size_t lds = 0;
     const void *fn = rpass_rf_kernel<T>(h->n1, h->has_mix, h->rp_lw, &lds);
     void *args[] = {&a};
     const int lw = h->rp_lw;
     const int nthreads = lw * rf_n1(h->n1);
-    const unsigned nblk = h->has_mix ? (unsigned)h->nbundle
-                                     : (unsigned)std::min(h->rp_slots, ((long long)h->nfcol * h->n2 + lw - 1) / lw);
+    const unsigned nblk = (unsigned)std::min(h->rp_slots, h->has_mix ? (long long)h->nbundle
+                                                                     : ((long long)h->nfcol * h->n2 + lw - 1) / lw);
     QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(nthreads), args, lds, h->stream));
     QD_HIP(hipGetLastError());
     return QD_OK;
@@ -1127,6 +1133,24 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   QD_RC(put_table(h, &h->twR, twR));
   QD_RC(put_table(h, &h->twr, twr));
   QD_RC(upload(&h->fcol_lam, fcol_lam.data(), fcol_lam.size()));
+  if (h->rp_rf && h->has_mix) {
+    // per-bundle line tables of the mixing R pass: element offset and D_R row per line
+    std::vector<long long> loff;
+    std::vector<int> llam, bl0;
+    for (int b = 0; b < h->nbundle; b++) {
+      bl0.push_back((int)loff.size());
+      for (int u = bu0[b]; u < bu0[b + 1]; u++)
+        for (int k = 0; k < units[u].g; k++) {
+          const int col = ucols[units[u].col0 + k];
+          loff.push_back(((long long)col * n2 + units[u].r) * n1);
+          llam.push_back(fcol_lam[col]);
+        }
+    }
+    bl0.push_back((int)loff.size());
+    QD_RC(upload(&h->bundle_l0, bl0.data(), bl0.size()));
+    QD_RC(upload(&h->line_off, loff.data(), loff.size()));
+    QD_RC(upload(&h->line_lam, llam.data(), llam.size()));
+  }
   QD_RC(upload(&h->fcol_j, S.fcol_j.data(), S.fcol_j.size()));
 
   // theta matrices, packed per set
@@ -1347,7 +1371,8 @@ void release(qd_handle *h) {
                   h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d1,     h->d2,       h->d1f,
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
-                  h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial};
+                  h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
+                  h->bundle_l0, h->line_off, h->line_lam};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : h->ev)

@@ -387,6 +387,11 @@ struct RPassArgs {
   int flags;
   FftPlanArg plan;
   int nfcol;                           // FBR columns (lines = nfcol * n2)
+  // register-FFT mixing kernel: per-bundle line tables (built on the host)
+  const int *bundle_l0;                // [nbundle + 1] first line of each bundle
+  const long long *line_off;           // element offset of each line
+  const int *line_lam;                 // D_R row of each line
+  int nbundle;
 };
 
 template <typename T>

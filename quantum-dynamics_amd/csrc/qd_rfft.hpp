@@ -356,100 +356,98 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
       buf ^= 1;
     }
   } else {
-    // A bundle of whole mixing units (the K blocks of one (J, j, r)) is
-    // staged in LDS:  load -> K-to-l mixing -> D_R/2, FFT, kinetic, inverse
-    // FFT, D_R/2 (register stages; the phase rows are read with the points)
-    // -> l-to-K mixing -> store.  The unit matrices and descriptors are
-    // copied to LDS first, so the mixing loops touch no global memory.
+    // Persistent over bundles of whole mixing units (the K blocks of one
+    // (J, j, r)), staged in LDS:  load -> K-to-l mixing -> D_R/2, FFT,
+    // kinetic, inverse FFT, D_R/2 (register stages; the phase rows are read
+    // with the points) -> l-to-K mixing -> store.  Line offsets and D_R rows
+    // come from host-built tables; the unit matrices are copied to LDS, so
+    // the mixing loops touch no global memory.
     __shared__ long long s_off[LW];
     __shared__ int s_lam[LW];
     __shared__ int s_unit_line0[LW + 1];
-    __shared__ int s_ug[LW], s_uoff[LW + 1];
+    __shared__ int s_ug[LW], s_mg[LW], s_usrc[LW], s_uoff[LW + 1];
     // U (row K, column m) and its transpose per mixed unit, padded to G4 = 4 ceil(g/4)
     __shared__ __align__(32) T s_um[LW * 40], s_umt[LW * 40];
-    const int ub = A.bundle_u0[blockIdx.x], ue = A.bundle_u0[blockIdx.x + 1];
-    const int nu = ue - ub;
-    const bool mix = (A.flags & FFT_MIX) != 0;
-    if (t == 0) {
-      int nl = 0, no = 0;
-      for (int u = 0; u < nu; u++) {
-        const RUnit U = A.units[ub + u];
-        s_unit_line0[u] = nl;
-        nl += U.g;
-        const bool mixed = mix && U.u0 >= 0;
-        s_ug[u] = mixed ? U.g : 0;
-        s_uoff[u] = no;
-        const int g4 = (U.g + 3) & ~3;
-        no += mixed ? g4 * g4 : 0;
-      }
-      s_unit_line0[nu] = nl;
-      s_uoff[nu] = no;
-    }
-    __syncthreads();
-    const int nl = s_unit_line0[nu];
-    if (t < nl) {
-      int u = 0;
-      while (s_unit_line0[u + 1] <= t) u++;
-      const RUnit U = A.units[ub + u];
-      const int col = A.unit_cols[U.col0 + (t - s_unit_line0[u])];
-      s_off[t] = ((long long)col * A.n2 + U.r) * N;
-      s_lam[t] = A.fcol_lam[col];
-    }
-    for (int u = 0; u < nu; u++) {
-      const int g = s_ug[u];
-      if (g == 0) continue;
-      const int g4 = (g + 3) & ~3;
-      const T *src = A.umat + A.units[ub + u].u0;
-      for (int i = t; i < g4 * g4; i += NTH) {
-        const int r = i / g4, c = i % g4;
-        const bool in = r < g && c < g;
-        s_um[s_uoff[u] + i] = in ? src[r * g + c] : (T)0;
-        s_umt[s_uoff[u] + i] = in ? src[c * g + r] : (T)0;
-      }
-    }
-    __syncthreads();
-    // fixed trip count: all of a thread's loads are in flight together
     constexpr int PER = (LW * N + NTH - 1) / NTH;
-    {
+    const bool mix = (A.flags & FFT_MIX) != 0;
+    for (int bdl = blockIdx.x; bdl < A.nbundle; bdl += gridDim.x) {
+      const int ub = A.bundle_u0[bdl], nu = A.bundle_u0[bdl + 1] - ub;
+      const int bl0 = A.bundle_l0[bdl], nl = A.bundle_l0[bdl + 1] - bl0;
+      __syncthreads();  // the previous bundle is done with LDS (first time: tables)
+      if (t < nu) {
+        const RUnit U = A.units[ub + t];
+        s_ug[t] = U.g;
+        s_mg[t] = (mix && U.u0 >= 0) ? U.g : 0;
+        s_usrc[t] = U.u0;
+      }
+      if (t < nl) {
+        s_off[t] = A.line_off[bl0 + t];
+        s_lam[t] = A.line_lam[bl0 + t];
+      }
+      __syncthreads();
+      // all of a thread's loads in flight together, while the unit layout is built
       C x[PER];
 #pragma unroll
       for (int i = 0; i < PER; i++) {
         const int e = t + i * NTH;
         if (e < nl * N) x[i] = A.data[s_off[e / N] + e % N];
       }
+      if (t == 0) {
+        int l0 = 0, no = 0;
+        for (int u = 0; u < nu; u++) {
+          s_unit_line0[u] = l0;
+          l0 += s_ug[u];
+          s_uoff[u] = no;
+          const int g4 = (s_mg[u] + 3) & ~3;
+          no += g4 * g4;
+        }
+        s_unit_line0[nu] = l0;
+        s_uoff[nu] = no;
+      }
+      __syncthreads();
+      for (int i = t; i < s_uoff[nu]; i += NTH) {
+        int u = 0;
+        while (s_uoff[u + 1] <= i) u++;
+        const int g = s_mg[u], g4 = (g + 3) & ~3, ii = i - s_uoff[u];
+        const int r = ii / g4, c = ii % g4;
+        const T *src = A.umat + s_usrc[u];
+        const bool in = r < g && c < g;
+        s_um[i] = in ? src[r * g + c] : (T)0;
+        s_umt[i] = in ? src[c * g + r] : (T)0;
+      }
 #pragma unroll
       for (int i = 0; i < PER; i++) {
         const int e = t + i * NTH;
         if (e < nl * N) stg[(e / N) * SP + e % N] = x[i];
       }
-    }
-    __syncthreads();
-    // K -> l: c_m = sum_K U[K, m] c_K
-    if (mix) {
-      rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_ug, s_unit_line0, s_uoff, s_um);
       __syncthreads();
-    }
-    const bool act = L < nl;
-    const TB *ph = A.dR + (size_t)(act ? s_lam[L] : 0) * N;
-    if (act) {
+      // K -> l: c_m = sum_K U[K, m] c_K
+      if (mix) {
+        rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_mg, s_unit_line0, s_uoff, s_um);
+        __syncthreads();
+      }
+      const bool act = L < nl;
+      const TB *ph = A.dR + (size_t)(act ? s_lam[L] : 0) * N;
+      if (act) {
 #pragma unroll
-      for (int m = 0; m < N2; m++) v[m] = tmul(stg[L * SP + a + N1 * m], ph[a + N1 * m], false);
-    }
-    rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
-    if (act) {
+        for (int m = 0; m < N2; m++) v[m] = tmul(stg[L * SP + a + N1 * m], ph[a + N1 * m], false);
+      }
+      rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
+      if (act) {
 #pragma unroll
-      for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
-    }
-    __syncthreads();
-    // l -> K: c_K = sum_m U[K, m] c_m
-    if (mix) {
-      rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_ug, s_unit_line0, s_uoff, s_umt);
+        for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+      }
       __syncthreads();
-    }
+      // l -> K: c_K = sum_m U[K, m] c_m
+      if (mix) {
+        rf_mix_bundle<T, N, SP, NTH>(stg, nu, s_mg, s_unit_line0, s_uoff, s_umt);
+        __syncthreads();
+      }
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int e = t + i * NTH;
-      if (e < nl * N) A.data[s_off[e / N] + e % N] = stg[(e / N) * SP + e % N];
+      for (int i = 0; i < PER; i++) {
+        const int e = t + i * NTH;
+        if (e < nl * N) A.data[s_off[e / N] + e % N] = stg[(e / N) * SP + e % N];
+      }
     }
   }
 }
