@@ -199,6 +199,28 @@ int qd_debug_set_fbr(qd_handle *h, const double *in);
 /* Last error message of the calling thread (empty string if none). */
 const char *qd_last_error(void);
 
+/* ---- learned J interpolation ------------------------------------------------
+ * No counterpart in the reference, which sums P^J(E) over propagated J only.
+ * A 3 x 128 tanh MLP in FP32 (inputs n_in <= 4, e.g. J, E and the reduced
+ * centrifugal barrier; one output, P^J(E)), trained on the GPU by hand-written
+ * forward, backward and Adam kernels on P^J(E) of the propagated J, used to
+ * predict P^J(E) of J that are skipped.  Parameters are one flat FP32 vector:
+ * W1[128][n_in], b1, W2[128][128], b2, W3[128][128], b3, w4[128], b4. */
+typedef struct qd_mlp qd_mlp;
+int qd_mlp_n_params(int32_t n_in, int64_t *n_out);
+int qd_mlp_create(int32_t n_in, const float *params, int32_t device, qd_mlp **out);
+/* training set: X[n][n_in] features, T[n] targets */
+int qd_mlp_set_data(qd_mlp *h, const float *X, const float *T, int32_t n);
+/* mean squared error and its gradient at the current parameters (no update) */
+int qd_mlp_loss_grad(qd_mlp *h, float *loss, float *grad);
+/* iters full-batch Adam steps (b1 0.9, b2 0.999, eps 1e-8); use_graph = 1
+ * replays one captured iteration as a hipGraph; last_loss may be NULL */
+int qd_mlp_train(qd_mlp *h, int32_t iters, float lr, int32_t use_graph, float *last_loss);
+int qd_mlp_predict(qd_mlp *h, const float *X, int32_t n, float *Y);
+int qd_mlp_get_params(qd_mlp *h, float *params);
+int qd_mlp_destroy(qd_mlp *h);
+const char *qd_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

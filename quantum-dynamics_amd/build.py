@@ -5,7 +5,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "qd_api.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("qd_api.hip", "qd_mlp.hip")]
 OUT = os.path.join(HERE, "libqd_hip.so")
 
 HIPCC_FLAGS = [
@@ -18,7 +18,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, SRC]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + SRCS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])

@@ -540,5 +540,8 @@ def make_jk_problem(base: Problem, system: LepsSystem, Js, jmax: int, j0: int = 
             th = Pbm[K0][j0 - K0]
         blk = g[:, None, None] * phi[None, :, None] * th[None, None, :]
         psi[b.offset:b.offset + blk.size] = np.transpose(blk, (2, 1, 0)).reshape(-1)
-    return JKProblem(base=base, jmax=jmax, Js=list(Js), blocks=blocks, x=x, omega=om, Pb=Pbm, Wf=Wfm,
+    # P(E) is normalised by |eta(E)|^2 of this packet (not of base's)
+    jbase = base.copy()
+    jbase.eta_sq = eta_sq(delta, k0, muR, base.energies - e_v)
+    return JKProblem(base=jbase, jmax=jmax, Js=list(Js), blocks=blocks, x=x, omega=om, Pb=Pbm, Wf=Wfm,
                      pot=pot, psi=psi, coriolis=coriolis)

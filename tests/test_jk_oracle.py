@@ -102,7 +102,7 @@ def test_jk_oracle_j0_is_the_reference_step(qd, oracle, jko, small_jk):
     jk = pb.make_jk_problem(base, pb.H_H2, [0], jmax=5)
     steps = 6
     psi, PJ, _ = jko.propagate(jk, steps)
-    p = base.copy()
+    p = jk.base.copy()
     assert p.ntheta == jk.jmax + 1 and p.m == jk.jmax
     p.pot = jk.pot[0].copy()
     p.psi = np.ascontiguousarray(jk.psi).view(np.float64).copy()
