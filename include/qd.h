@@ -113,12 +113,14 @@ typedef struct qd_options {
   int32_t variant;        /* QD_VARIANT_* */
   int32_t flux_block;     /* steps per time block of the flux accumulation; 0 = 64 */
   int32_t crp_every;      /* >0: evaluate P(E) (and call the callback) every N steps */
-  int32_t use_graph;      /* 1: replay captured per-block step graphs */
+  int32_t use_graph;      /* 1: run whole flux blocks (flux_block steps and their
+                             accumulation) as one replayed hipGraph */
   int32_t device;         /* HIP device ordinal */
   int32_t profile;        /* 1: time every kernel class with HIP events (qd_get_timing) */
   int32_t theta_split;    /* mixed theta GEMM: bf16 pieces per operand, 2 (hi+lo, three
                              products) or 3 (six products); 0 = 3 */
-  int32_t reserved;
+  int32_t autotune;       /* 1: time the candidate kernel configurations on the
+                             handle's own arrays at creation and keep the fastest */
 } qd_options;
 
 /* Per-step callback, the equivalent of options.wave_to_matlab

@@ -161,8 +161,13 @@ struct qd_handle {
   // state
   int32_t steps = 0;
   bool pending_prephase = false;
-  long long acc_steps = 0;
+  long long acc_steps = 0;            // host mirror of d_acc_steps
+  long long *d_acc_steps = nullptr;   // steps already accumulated into fai (device)
   int slot = 0;
+  // captured flux block (tblock steps + accumulation) for qd_options.use_graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int use_graph = 0;
   qd_step_cb cb = nullptr;
   void *cb_user = nullptr;
   hipStream_t stream = nullptr;
@@ -524,7 +529,8 @@ int flush_flux(qd_handle *h) {
   QD_RC(mark(h, TK_FLUX));
   dim3 grid((unsigned)((h->nx + FX_X - 1) / FX_X), (unsigned)((h->ne + FX_E - 1) / FX_E));
   hipLaunchKernelGGL(k_flux_accumulate, grid, dim3(256), 0, h->stream, h->bpsi, h->bdpsi, h->fai, h->dfai,
-                     h->energies, (int)h->nx, h->ne, h->slot, h->acc_steps, h->dt);
+                     h->energies, (int)h->nx, h->ne, h->slot, h->d_acc_steps, h->dt);
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, h->stream, h->d_acc_steps, h->slot);
   QD_HIP(hipGetLastError());
   h->acc_steps += h->slot;
   h->slot = 0;
@@ -661,6 +667,7 @@ int set_options(qd_handle *h, const qd_options *o) {
   h->pipe = opt.pipeline;
   h->variant = opt.variant;
   h->tblock = opt.flux_block > 0 ? opt.flux_block : 64;
+  h->use_graph = opt.use_graph;
   h->crp_every = opt.crp_every;
   h->profile = opt.profile;
   h->nsplit = opt.theta_split == 2 ? 2 : 3;
@@ -1205,6 +1212,11 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_RC(upload(&h->energies, p->crp.energies, h->ne));
     QD_RC(upload(&h->eta_sq, p->crp.eta_sq, h->ne));
     QD_HIP(hipMalloc((void **)&h->crp, sizeof(double) * h->ne * h->nb));
+    QD_HIP(hipMalloc((void **)&h->d_acc_steps, sizeof(long long)));
+    {
+      const long long z = 0;
+      QD_HIP(hipMemcpy(h->d_acc_steps, &z, sizeof(z), hipMemcpyHostToDevice));
+    }
     QD_HIP(hipMalloc((void **)&h->bpsi, sizeof(double2) * h->nx * h->tblock));
     QD_HIP(hipMalloc((void **)&h->bdpsi, sizeof(double2) * h->nx * h->tblock));
     QD_HIP(hipMalloc((void **)&h->fai, sizeof(double2) * h->nx * h->ne));
@@ -1372,7 +1384,9 @@ void release(qd_handle *h) {
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
-                  h->bundle_l0, h->line_off, h->line_lam};
+                  h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps};
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : h->ev)
@@ -1380,6 +1394,30 @@ void release(qd_handle *h) {
   for (auto &e : h->marks)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Captures tblock steps and the flux accumulation that closes the block.  The
+// host-side block state advanced by the capture is put back: replays advance
+// it (propagate_impl) and the device counter (k_counter_add).
+int capture_block(qd_handle *h) {
+  const int slot0 = h->slot;
+  const long long acc0 = h->acc_steps;
+  QD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = QD_OK;
+  for (int k = 0; k < h->tblock && rc == QD_OK; k++)
+    rc = h->prec == QD_PREC_FP64 ? step_impl<double>(h) : step_impl<float>(h);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->slot = slot0;
+  h->acc_steps = acc0;
+  if (rc != QD_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  QD_HIP(e);
+  h->graph = g;
+  QD_HIP(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  return QD_OK;
 }
 
 int propagate_impl(qd_handle *h, int32_t nsteps) {
@@ -1397,10 +1435,28 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
   hipEvent_t e0 = h->ev[0], e1 = h->ev[1];
   h->nmarks = 0;
   QD_HIP(hipEventRecord(e0, h->stream));
-  for (int s = 0; s < nsteps; s++) {
-    if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h));
-    else QD_RC(step_impl<float>(h));
-    h->steps++;
+  const bool graphs = h->use_graph && !h->profile;
+  for (int s = 0; s < nsteps;) {
+    // a whole flux block as one graph replay when it starts a block and no
+    // P(E) / callback point falls inside it; otherwise one step eagerly
+    const int B = h->tblock;
+    bool replay = graphs && h->slot == 0 && nsteps - s >= B;
+    if (replay && h->crp_every > 0 && (h->cb || h->do_crp)) {
+      const long long next = (h->steps / h->crp_every + 1) * h->crp_every;
+      replay = next >= h->steps + B;
+    }
+    if (replay) {
+      if (!h->gexec) QD_RC(capture_block(h));
+      QD_HIP(hipGraphLaunch(h->gexec, h->stream));
+      h->steps += B;
+      s += B;
+      if (h->do_crp) h->acc_steps += B;
+    } else {
+      if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h));
+      else QD_RC(step_impl<float>(h));
+      h->steps++;
+      s++;
+    }
     const bool every = h->crp_every > 0 && (h->steps % h->crp_every == 0);
     if (every && (h->cb || h->do_crp)) {
       std::vector<double> tot;

@@ -1156,11 +1156,18 @@ constexpr int FX_E = 32;
 constexpr int FX_EPT = 8;
 constexpr int FX_TC = 16;
 
+// counter += add (one thread): advances device-side step counts between
+// kernels, so a captured graph can be replayed without new arguments
+__global__ void k_counter_add(long long *counter, int add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter += add;
+}
+
 __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restrict__ bpsi,
                                                          const double2 *__restrict__ bdpsi,
                                                          double2 *__restrict__ fai, double2 *__restrict__ dfai,
                                                          const double *__restrict__ energies, int nx, int ne,
-                                                         int nt, long long n0, double dt) {
+                                                         int nt, const long long *__restrict__ n0p, double dt) {
+  const long long n0 = *n0p;  // steps accumulated before this block (device counter)
   __shared__ double2 sb[FX_TC][FX_X];
   __shared__ double2 sdb[FX_TC][FX_X];
   __shared__ double2 stau[FX_TC][FX_E];

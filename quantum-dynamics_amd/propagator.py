@@ -28,9 +28,9 @@ VARIANTS = {"cpu": _abi.QD_VARIANT_CPU, "cuda": _abi.QD_VARIANT_CUDA}
 
 
 def make_options(precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0, use_graph=0):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
     return _abi.qd_options(PRECISIONS[precision], PIPELINES[pipeline], VARIANTS[variant], flux_block,
-                           crp_every, use_graph, device, profile, theta_split, 0)
+                           crp_every, use_graph, device, profile, theta_split, autotune)
 
 
 def _field(d, name, required=True):
@@ -121,12 +121,12 @@ class Propagator:
     """Handle API over qd_create / qd_propagate / qd_get_* / qd_destroy."""
 
     def __init__(self, problem: Problem, precision="fp64", pipeline="fused", variant="cpu",
-                 flux_block=0, crp_every=0, device=0, profile=0, theta_split=0):
+                 flux_block=0, crp_every=0, device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
         self.lib = _abi.load_library()
         self.problem = problem
         self._st = _abi.ProblemStruct(problem)
         self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile,
-                                 theta_split)
+                                 theta_split, use_graph, autotune)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create(C.byref(self._st.s), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -248,7 +248,7 @@ class JKPropagator(Propagator):
     """Handle over a batch of (J, K) blocks (qd_create_jk)."""
 
     def __init__(self, jk, precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
         self.lib = _abi.load_library()
         self.jk = jk
         base = jk.base
@@ -274,7 +274,8 @@ class JKPropagator(Propagator):
         q.n_blocks = len(jk.blocks)
         q.blocks = C.cast(blocks, C.POINTER(_abi.qd_jk_block))
         self._q = q
-        self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile, theta_split)
+        self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile, theta_split,
+                                 use_graph, autotune)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create_jk(C.byref(q), C.byref(self._opt), C.byref(h)))
         self.h = h

@@ -69,3 +69,27 @@ def test_handle_resume_and_norm(qd, cfg0):
         prop.propagate(15)
         assert rel_psi(prop.psi(), psi_a) < 1e-14
         assert crp_err(prop.crp(), crp_a) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("crp_every", [0, 20])
+def test_graph_replay_equals_eager(qd, cfg0, prec, crp_every):
+    """Whole flux blocks replayed as one captured graph give the same psi and
+    P(E) as the same kernels launched one by one.  Three propagate calls with
+    host work between them replay the graph several times; with crp_every = 20
+    some blocks hold a P(E) point and run eagerly, so both kinds interleave."""
+    print("case: graph replay", prec, "crp_every", crp_every, flush=True)
+    out = []
+    for graph in (0, 1):
+        with qd.Propagator(cfg0.copy(), precision=prec, flux_block=16, crp_every=crp_every,
+                           use_graph=graph) as prop:
+            seen = []
+            for n in (40, 33, 55):
+                prop.propagate(n)
+                seen.append(prop.crp().copy())       # eager work and a host sync between replays
+            out.append((prop.psi(), seen, prop.steps()))
+    (pa, ca, sa), (pb, cb, sb) = out
+    assert sa == sb == 128
+    assert np.array_equal(pa, pb)
+    for a, b in zip(ca, cb):
+        assert np.array_equal(a, b)
