@@ -36,9 +36,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per grid point of one launch of each kernel class
-# (68 B per grid-point-step mixed, 136 B FP64):
-# theta-fwd reads psi (s_c) and V (s_v) and writes the FBR array; the R and r
-# passes and theta-bwd each read and write one complex array.
+# (B = s_c (2 + 6 M / n_theta) + s_v per grid-point-step: 68 B mixed, 136 B
+# FP64): theta-fwd reads psi (s_c) and V (s_v) and writes the FBR array; the R
+# and r passes and theta-bwd each read and write one complex array.
 B_ALG = {
     "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "fft_R": 16, "fft_r": 16, "total": 68},
     "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
@@ -60,19 +60,21 @@ def _run(prop, steps, warmup, profile):
     t0 = time.perf_counter()
     prop.propagate(steps)          # returns after the device has finished
     wall = time.perf_counter() - t0
-    out = {"wall_ms_per_step": 1e3 * wall / steps, "event_ms_per_step": prop.timing(0)}
+    out = {"wall_ms_per_step": 1e3 * wall / steps, "event_ms_per_step": prop.timing(0),
+           "tuning": prop.tuning()}
     if profile:
         out["kernel_ms"] = {k: prop.timing(i) for k, i in KCLASSES}
     return out
 
 
-def timed_single(qd, prob, precision, pipeline, steps, warmup, profile=0):
-    with qd.Propagator(prob.copy(), precision=precision, pipeline=pipeline, profile=profile) as prop:
+def timed_single(qd, prob, precision, pipeline, steps, warmup, profile=0, autotune=0):
+    with qd.Propagator(prob.copy(), precision=precision, pipeline=pipeline, profile=profile,
+                       autotune=autotune) as prop:
         return _run(prop, steps, warmup, profile)
 
 
-def timed_batched(qd, jk, precision, steps, warmup, profile=0):
-    with qd.propagator.JKPropagator(jk, precision=precision, profile=profile) as prop:
+def timed_batched(qd, jk, precision, steps, warmup, profile=0, autotune=0):
+    with qd.propagator.JKPropagator(jk, precision=precision, profile=profile, autotune=autotune) as prop:
         return _run(prop, steps, warmup, profile)
 
 
@@ -162,7 +164,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagator runs on a single MI355X here; the batched "
-              "workload would shard by J across GPUs (DESIGN.md section 6)", file=sys.stderr)
+              "workload would shard by J across GPUs (the K blocks of one J stay together; P^J(E) is "
+              "gathered once at the end)", file=sys.stderr)
         sys.exit(2)
     qd = importlib.import_module("quantum-dynamics_amd")
     K, W = args.steps, args.warmup
@@ -170,8 +173,9 @@ def main():
     # single grid (F+H2 J=0, 256 x 192 x 120)
     prob = qd.problem.config(2)
     npt1 = prob.grid_points
-    s_mixed = timed_single(qd, prob, "mixed", "fused", K, W)
-    s_mixed.update(timed_single(qd, prob, "mixed", "fused", max(20, K // 4), 5, profile=1))
+    s_mixed = timed_single(qd, prob, "mixed", "fused", K, W, autotune=1)
+    s_mixed["kernel_ms"] = timed_single(qd, prob, "mixed", "fused", max(20, K // 4), 5, profile=1,
+                                        autotune=1)["kernel_ms"]
     s_f64 = timed_single(qd, prob, "fp64", "fused", max(20, K // 2), 5, profile=1)
     s_plain = timed_single(qd, prob, "fp64", "plain", max(20, K // 2), 5)
     single = {
@@ -188,8 +192,8 @@ def main():
         npt = jk.grid_points
         label = ("F+H2 J=0..20, K=0..min(J,60), j_max=60: 231 Coriolis-coupled (J,K) blocks of 256x192x(61-K), "
                  "mixed precision")
-        m = timed_batched(qd, jk, "mixed", K, W)
-        prof = timed_batched(qd, jk, "mixed", max(10, K // 10), 2, profile=1)
+        m = timed_batched(qd, jk, "mixed", K, W, autotune=1)
+        prof = timed_batched(qd, jk, "mixed", max(10, K // 10), 2, profile=1, autotune=1)
         f64 = timed_batched(qd, jk, "fp64", max(10, K // 10), 2)
         plain_ms, plain_per_n, counts = plain_fp64_same_grids(qd, jk, max(10, K // 10), 3)
         step_ms = m["wall_ms_per_step"]
@@ -233,6 +237,7 @@ def main():
         "fused_fp64": fused,
         "speedup_vs_plain_fp64": value / plain["value"],
         "kernel_ms": kern,
+        "tuning": m["tuning"],
         "roofline": rf,
         "single_grid": single,
         "cpu_baseline": None,

@@ -195,6 +195,11 @@ int qd_get_crp_blocks(qd_handle *h, double *crp_out);
  * 5: forward FFT along r only; 6: inverse FFT along R only; 7: inverse FFT
  * along r only.  The FBR array has n1*n2*(FBR columns) complex values. */
 int qd_debug_stage(qd_handle *h, int32_t stage);
+/* kernel configuration in use: [0] autotuned, [1] theta k-steps of the
+ * matrix-stationary kernel (0: tiled kernel), [2] its 16-row tiles per work
+ * group, [3] register-FFT R pass, [4] register-FFT r pass, [5] R-pass lines
+ * per work group, [6] lines per mixing bundle, [7] r-pass lines per group */
+int qd_get_tuning(qd_handle *h, int32_t *vals);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);
 

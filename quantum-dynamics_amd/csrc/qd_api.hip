@@ -168,6 +168,7 @@ struct qd_handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int use_graph = 0;
+  bool tuned = false;
   qd_step_cb cb = nullptr;
   void *cb_user = nullptr;
   hipStream_t stream = nullptr;
@@ -1518,11 +1519,74 @@ int norm_impl(qd_handle *h, double *out) {
   return QD_OK;
 }
 
+// ---- autotuner ----
+// Times each candidate configuration of a pass on the handle's own arrays
+// (theta forward reads psi and writes the FBR scratch array; the R and r
+// passes work in place on the FBR array, which the next step overwrites) and
+// keeps the fastest.  psi is not modified.
+template <typename F>
+int time_launches(qd_handle *h, F launch, int reps, float *ms) {
+  QD_RC(launch());  // warm-up (and first-use setup)
+  QD_HIP(hipEventRecord(h->ev[0], h->stream));
+  for (int k = 0; k < reps; k++) QD_RC(launch());
+  QD_HIP(hipEventRecord(h->ev[1], h->stream));
+  QD_HIP(hipEventSynchronize(h->ev[1]));
+  QD_HIP(hipEventElapsedTime(ms, h->ev[0], h->ev[1]));
+  *ms /= reps;
+  return QD_OK;
+}
+
+int autotune_handle(qd_handle *h) {
+  const int reps = 3;
+  const int full = FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV;
+  const bool f64 = h->prec == QD_PREC_FP64;
+  void *lpsi = f64 ? (void *)h->lpsi64 : (void *)h->lpsi32;
+  // theta: 16-row tiles per work group of the matrix-stationary kernel
+  if (h->tmx_ks) {
+    const long long ntiles = (h->n12 + 15) / 16;
+    const int base = h->tmx_tpw;
+    std::vector<int> cands = {base, std::max(1, base / 2), base * 3 / 2, base * 2, base * 4};
+    float best = 1e30f;
+    int pick = base;
+    for (int c : cands) {
+      if (c < 1 || c > ntiles) continue;
+      h->tmx_tpw = c;
+      float ms = 0;
+      QD_RC(time_launches(h, [&] { return launch_theta(h, true, 1, 0); }, reps, &ms));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    h->tmx_tpw = pick;
+  }
+  // R and r passes: register FFT or Stockham kernels
+  auto pick_pass = [&](bool *flag, bool alongR) -> int {
+    if (!*flag) return QD_OK;
+    float t_rf = 0, t_st = 0;
+    auto run = [&] {
+      if (alongR)
+        return f64 ? launch_rpass<double>(h, full, (double2 *)lpsi) : launch_rpass<float>(h, full, (float2 *)lpsi);
+      return f64 ? launch_rline<double>(h, full, (double2 *)lpsi) : launch_rline<float>(h, full, (float2 *)lpsi);
+    };
+    QD_RC(time_launches(h, run, reps, &t_rf));
+    *flag = false;
+    QD_RC(time_launches(h, run, reps, &t_st));
+    *flag = t_rf <= t_st;
+    return QD_OK;
+  };
+  QD_RC(pick_pass(&h->rp_rf, true));
+  QD_RC(pick_pass(&h->rl_rf, false));
+  h->tuned = true;
+  return QD_OK;
+}
+
 int create_from_spec(const BatchSpec &S, const qd_options *options, qd_handle **out) {
   qd_handle *h = new (std::nothrow) qd_handle;
   if (!h) return fail(QD_ERR_NOMEM, "out of host memory");
   int rc = set_options(h, options);
   if (rc == QD_OK) rc = create_batch(h, S);
+  if (rc == QD_OK && options && options->autotune) rc = autotune_handle(h);
   if (rc != QD_OK) {
     release(h);
     delete h;
@@ -1552,6 +1616,19 @@ int qd_create_jk(const qd_jk_problem *problem, const qd_options *options, qd_han
   BatchSpec S;
   QD_RC(spec_from_jk(problem, S));
   return create_from_spec(S, options, out);
+}
+
+int qd_get_tuning(qd_handle *h, int32_t *vals) {
+  if (!h || !vals) return fail(QD_ERR_NULL, "NULL argument");
+  vals[0] = h->tuned ? 1 : 0;
+  vals[1] = h->tmx_ks;
+  vals[2] = h->tmx_tpw;
+  vals[3] = h->rp_rf ? 1 : 0;
+  vals[4] = h->rl_rf ? 1 : 0;
+  vals[5] = h->rp_lw;
+  vals[6] = h->rp_ml;
+  vals[7] = h->rl_lpb;
+  return QD_OK;
 }
 
 int qd_jk_theta_grid(int32_t K, int32_t jmax, double *x, double *w) {

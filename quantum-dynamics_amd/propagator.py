@@ -168,6 +168,14 @@ class Propagator:
         _abi.check(self.lib.qd_get_steps(self.h, C.byref(v)))
         return v.value
 
+    def tuning(self) -> dict:
+        """Kernel configuration in use (qd_get_tuning)."""
+        v = (C.c_int32 * 8)()
+        _abi.check(self.lib.qd_get_tuning(self.h, v))
+        keys = ("autotuned", "theta_ksteps", "theta_tiles_per_group", "rpass_register_fft",
+                "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group")
+        return dict(zip(keys, list(v)))
+
     def timing(self, which: int) -> float:
         v = C.c_double()
         _abi.check(self.lib.qd_get_timing(self.h, int(which), C.byref(v)))

@@ -93,3 +93,25 @@ def test_graph_replay_equals_eager(qd, cfg0, prec, crp_every):
     assert np.array_equal(pa, pb)
     for a, b in zip(ca, cb):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_autotuned_handle_matches(qd, oracle, cfg0, prec):
+    """The autotuner only picks among equivalent kernels: an autotuned FP64
+    handle still matches the oracle, an autotuned mixed one the default."""
+    print("case: autotune", prec, flush=True)
+    steps = 40
+    with qd.Propagator(cfg0.copy(), precision=prec, autotune=1) as prop:
+        tun = prop.tuning()
+        prop.propagate(steps)
+        psi_t, crp_t = prop.psi(), prop.crp()
+    print("tuning", tun)
+    assert tun["autotuned"] == 1
+    if prec == "fp64":
+        psi_o, crp_o, _, _ = oracle.time_evolution(cfg0, total_steps=steps)
+        assert rel_psi(psi_t, psi_o) < 1e-10
+        assert crp_err(crp_t, crp_o) < 1e-8
+    else:
+        psi_d, crp_d = qd.run(cfg0, steps=steps, precision="mixed")
+        assert rel_psi(psi_t, psi_d) < 1e-5
+        assert np.max(np.abs(crp_t - crp_d)) < 1e-5
