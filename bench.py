@@ -158,9 +158,24 @@ def main():
     ap.add_argument("--workload", choices=("batched", "single"), default="batched")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a device-to-device copy of known size (HBM counter calibration)")
+    ap.add_argument("--only", choices=("batched-mixed", "single-mixed"),
+                    help="run just this step (autotuned) for K steps after W warm-up steps and print its "
+                         "timing; for profiler runs")
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
+        return
+    if args.only:
+        qd = importlib.import_module("quantum-dynamics_amd")
+        if args.only == "batched-mixed":
+            jk = batched_problem(qd)
+            r = timed_batched(qd, jk, "mixed", args.steps, args.warmup, profile=1, autotune=1)
+            npt = jk.grid_points
+        else:
+            prob = qd.problem.config(2)
+            r = timed_single(qd, prob, "mixed", "fused", args.steps, args.warmup, profile=1, autotune=1)
+            npt = prob.grid_points
+        print(json.dumps({"only": args.only, "grid_points": npt, "steps": args.steps, **r}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagator runs on a single MI355X here; the batched "

@@ -198,7 +198,9 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
 /* kernel configuration in use: [0] autotuned, [1] theta k-steps of the
  * matrix-stationary kernel (0: tiled kernel), [2] its 16-row tiles per work
  * group, [3] register-FFT R pass, [4] register-FFT r pass, [5] R-pass lines
- * per work group, [6] lines per mixing bundle, [7] r-pass lines per group */
+ * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
+ * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
+ * [9] its unit-size bucket.  vals holds 10 entries. */
 int qd_get_tuning(qd_handle *h, int32_t *vals);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

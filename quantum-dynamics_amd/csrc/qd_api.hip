@@ -30,6 +30,7 @@
 #include "qd_fft.hpp"
 #include "qd_rfft.hpp"
 #include "qd_kernels.hpp"
+#include "qd_dispatch.hpp"
 #include "qd_tables.hpp"
 
 using namespace qd;
@@ -200,6 +201,15 @@ struct qd_handle {
   int rp_lw = 0;
   long long rp_slots = 1, rl_slots = 1;  // resident work groups of the persistent passes
   int *bundle_l0 = nullptr, *line_lam = nullptr;  // mixing R pass: per-bundle line tables
+  // FP32 mixing R pass with prefetch (k_rpass_mx): bundle records
+  bool rp_mx = false, rp_legacy = true;  // legacy: the LDS-staged and Stockham R passes are set up
+  int mx_nl = 0, mx_gb = 0, mx_maxu = 0, mx_nbundle = 0;
+  long long mx_slots = 1;
+  size_t mx_lds = 0;
+  MxHead *mx_head = nullptr;
+  long long *mx_off = nullptr;
+  int *mx_lam = nullptr, *mx_unit = nullptr;
+  float *mx_umat = nullptr;
   long long *line_off = nullptr;
   size_t tmx_lds = 0;
   bool rp_ct = false, rl_ct = false;
@@ -260,103 +270,6 @@ int mark(qd_handle *h, int cls) {
   return QD_OK;
 }
 
-// ---- compile-time FFT dispatch ----
-#define QD_CT_LENGTHS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
-
-bool ct_length(int n) {
-  switch (n) {
-#define QD_CASE(N) case N:
-    QD_CT_LENGTHS(QD_CASE)
-#undef QD_CASE
-    return true;
-    default:
-      return false;
-  }
-}
-
-template <typename T, int N, int ML>
-const void *rpass_ct_fn(bool mix) {
-  if constexpr (256 / ML * FFT_BPT >= ct_maxnb(N))
-    return mix ? (const void *)k_rpass_ct<T, N, ML, true> : (const void *)k_rpass_ct<T, N, ML, false>;
-  else return nullptr;
-}
-template <typename T, int N, int LPB>
-const void *rline_ct_fn() {
-  if constexpr (256 / LPB * FFT_BPT >= ct_maxnb(N)) return (const void *)k_rline_ct<T, N, LPB>;
-  else return nullptr;
-}
-// kernel for (length, lines per group); nullptr -> generic kernel
-template <typename T>
-const void *rpass_kernel(int n, int ml, bool mix) {
-  switch (n) {
-#define QD_CASE(N) \
-  case N:          \
-    return ml == 8 ? rpass_ct_fn<T, N, 8>(mix) : ml == 32 ? rpass_ct_fn<T, N, 32>(mix) : nullptr;
-    QD_CT_LENGTHS(QD_CASE)
-#undef QD_CASE
-    default:
-      return nullptr;
-  }
-}
-template <typename T>
-const void *rline_kernel(int n, int lpb) {
-  switch (n) {
-#define QD_CASE(N) \
-  case N:          \
-    return lpb == 16 ? rline_ct_fn<T, N, 16>() : lpb == 8 ? rline_ct_fn<T, N, 8>() : nullptr;
-    QD_CT_LENGTHS(QD_CASE)
-#undef QD_CASE
-    default:
-      return nullptr;
-  }
-}
-
-// register-FFT passes: lengths with a factorisation N1 x N2 (rf_n1)
-#define QD_RF_LENGTHS(X) X(24, 8) X(32, 8) X(48, 8) X(64, 8) X(96, 16) X(128, 16) X(192, 16) X(256, 16) X(384, 32) X(512, 32)
-template <typename T, int N1, int N2>
-const void *rpass_rf_fn(bool mix, int lw, size_t *lds) {
-  constexpr int LW0 = 256 / N1;
-  if (!mix) {
-    if (lw != LW0) return nullptr;
-    *lds = rpass_rf_lds<T, N1, N2, false, LW0>();
-    return (const void *)k_rpass_rf<T, N1, N2, false, LW0>;
-  }
-  if (lw == LW0) {
-    *lds = rpass_rf_lds<T, N1, N2, true, LW0>();
-    return (const void *)k_rpass_rf<T, N1, N2, true, LW0>;
-  }
-  if (lw == 2 * LW0) {  // 512 threads: mixing groups of up to 2 * LW0 lines
-    *lds = rpass_rf_lds<T, N1, N2, true, 2 * LW0>();
-    return (const void *)k_rpass_rf<T, N1, N2, true, 2 * LW0>;
-  }
-  return nullptr;
-}
-template <typename T>
-const void *rpass_rf_kernel(int n, bool mix, int lw, size_t *lds) {
-  switch (n) {
-#define QD_CASE(N, N1) \
-  case N:              \
-    return rpass_rf_fn<T, N1, N / N1>(mix, lw, lds);
-    QD_RF_LENGTHS(QD_CASE)
-#undef QD_CASE
-    default:
-      return nullptr;
-  }
-}
-template <typename T>
-const void *rline_rf_kernel(int n, size_t *lds, int *lpb) {
-  switch (n) {
-#define QD_CASE(N, N1)                             \
-  case N:                                          \
-    *lds = rline_rf_lds<T, N1, N / N1>();          \
-    *lpb = 256 / N1;                               \
-    return (const void *)k_rline_rf<T, N1, N / N1>;
-    QD_RF_LENGTHS(QD_CASE)
-#undef QD_CASE
-    default:
-      return nullptr;
-  }
-}
 constexpr int QD_RF_FLAGS = FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV;
 
 // ---- launches ----
@@ -385,6 +298,30 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.line_off = h->line_off;
   a.line_lam = h->line_lam;
   a.nbundle = h->nbundle;
+  if constexpr (sizeof(T) == 4) {
+    if (h->rp_mx && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
+      RMixArgs m;
+      m.data = (float2 *)data;
+      m.tw = (const float4 *)h->twR;
+      m.kin = (const float4 *)h->kinR;
+      m.dR = (const float4 *)h->dR;
+      m.head = h->mx_head;
+      m.line_off = h->mx_off;
+      m.line_lam = h->mx_lam;
+      m.unit = h->mx_unit;
+      m.umat = h->mx_umat;
+      m.nbundle = h->mx_nbundle;
+      m.maxu = h->mx_maxu;
+      size_t lds0 = 0;
+      const void *fn = rpass_mx_kernel(h->n1, h->mx_nl, h->mx_gb, &lds0);
+      void *args[] = {&m};
+      const unsigned nblk = (unsigned)std::min(h->mx_slots, (long long)h->mx_nbundle);
+      QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->mx_nl * rf_n1(h->n1)), args, h->mx_lds, h->stream));
+      QD_HIP(hipGetLastError());
+      return QD_OK;
+    }
+  }
+  if (!h->rp_legacy) return fail(QD_ERR_ARG, "this R-pass stage needs mixing groups of at most 32 lines");
   if (h->rp_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
     size_t lds = 0;
     const void *fn = rpass_rf_kernel<T>(h->n1, h->has_mix, h->rp_lw, &lds);
@@ -443,23 +380,6 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   }
   QD_HIP(hipGetLastError());
   return QD_OK;
-}
-
-// matrix-stationary mixed theta kernel for KS k-steps (KP = NP = 32 KS)
-const void *theta_mx_kernel(int ks, int pro, int epi) {
-#define QD_TMX(KS_)                                                                    \
-  if (ks == KS_) {                                                                     \
-    if (pro == 0 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 0>); \
-    if (pro == 1 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 1, 0>); \
-    if (pro == 2 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 2, 0>); \
-    if (pro == 0 && epi == 1) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 1>); \
-  }
-  QD_TMX(1)
-  QD_TMX(2)
-  QD_TMX(3)
-  QD_TMX(4)
-#undef QD_TMX
-  return nullptr;
 }
 
 int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
@@ -768,6 +688,115 @@ int setup_theta_mx(qd_handle *h) {
   return QD_OK;
 }
 
+// Bundle records of the FP32 mixing R pass (k_rpass_mx): units (group, r)
+// packed in order into bundles of at most NL lines and two distinct unit
+// matrices; per bundle a header and NL-strided line and unit entries.  The
+// matrices are stored padded to g4 = 4 ceil(g / 4), U then U^T, rounded to
+// FP32 with unit column lengths (qdt::round_orthogonal_columns).
+int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam, int gmax) {
+  const int n1 = h->n1, n2 = h->n2;
+  const int NL = gmax <= 32 ? 32 : 48;
+  const int GB = gmax <= 8 ? 8 : gmax <= 16 ? 16 : gmax <= 24 ? 24 : gmax <= 32 ? 32 : 48;
+  size_t lds_fixed = 0;
+  const void *fn = rpass_mx_kernel(n1, NL, GB, &lds_fixed);
+  if (!fn) return QD_OK;
+  std::vector<float> upad;
+  std::map<int, std::pair<int, int>> mat_of;  // group u0 -> (offset, g4)
+  for (const Group &g : S.groups) {
+    const int gs = (int)g.cols.size();
+    if (g.u0 < 0 || gs < 2 || mat_of.count(g.u0)) continue;
+    const int g4 = (gs + 3) & ~3;
+    std::vector<float> q;
+    qdt::round_orthogonal_columns(S.umat.data() + g.u0, gs, q);
+    const int off = (int)upad.size();
+    upad.resize(upad.size() + 2 * (size_t)g4 * g4, 0.f);
+    for (int K = 0; K < gs; K++)
+      for (int m = 0; m < gs; m++) {
+        upad[off + (size_t)K * g4 + m] = q[(size_t)K * gs + m];                  // U[K][m]
+        upad[off + (size_t)g4 * g4 + (size_t)m * g4 + K] = q[(size_t)K * gs + m];  // U^T[m][K]
+      }
+    mat_of[g.u0] = {off, g4};
+  }
+  std::vector<MxHead> heads;
+  std::vector<long long> loff;
+  std::vector<int> llam, uni;
+  MxHead cur{};
+  int maxu = 0;
+  auto flush = [&]() {
+    if (cur.nl == 0) return;
+    heads.push_back(cur);
+    maxu = std::max(maxu, cur.mg4[0] * cur.mg4[0] + cur.mg4[1] * cur.mg4[1]);
+    cur = MxHead{};
+  };
+  auto open_bundle = [&]() {
+    loff.resize(loff.size() + NL, 0);
+    llam.resize(llam.size() + NL, 0);
+    uni.resize(uni.size() + NL, 0);
+    cur.msrc[0] = cur.msrc[1] = 0;
+  };
+  for (const Group &g : S.groups) {
+    const int gs = (int)g.cols.size();
+    const bool mixed = g.u0 >= 0 && gs > 1;
+    const int src = mixed ? mat_of[g.u0].first : -1, g4 = mixed ? mat_of[g.u0].second : 0;
+    for (int r = 0; r < n2; r++) {
+      int slot = 0;
+      if (cur.nl > 0) {
+        bool fits = cur.nl + gs <= NL;
+        if (fits && mixed) {
+          const bool have = (cur.mg4[0] && cur.msrc[0] == src) || (cur.mg4[1] && cur.msrc[1] == src);
+          fits = have || cur.mg4[1] == 0;
+        }
+        if (!fits) flush();
+      }
+      if (cur.nl == 0) open_bundle();
+      if (mixed) {
+        if (cur.mg4[0] && cur.msrc[0] == src) slot = 1;
+        else if (cur.mg4[1] && cur.msrc[1] == src) slot = 2;
+        else if (!cur.mg4[0]) {
+          cur.msrc[0] = src;
+          cur.mg4[0] = g4;
+          slot = 1;
+        } else {
+          cur.msrc[1] = src;
+          cur.mg4[1] = g4;
+          slot = 2;
+        }
+      }
+      const size_t bb = heads.size() * (size_t)NL;
+      uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
+      for (int k = 0; k < gs; k++) {
+        const int col = g.cols[k];
+        loff[bb + cur.nl + k] = ((long long)col * n2 + r) * n1;
+        llam[bb + cur.nl + k] = fcol_lam[col];
+      }
+      cur.nl += gs;
+      cur.nu += 1;
+    }
+  }
+  flush();
+  const size_t lds = lds_fixed + 2 * (size_t)maxu * sizeof(float);
+  if (lds + (size_t)NL * 12 > 160 * 1024) return QD_OK;
+  QD_RC(upload(&h->mx_head, heads.data(), heads.size()));
+  QD_RC(upload(&h->mx_off, loff.data(), loff.size()));
+  QD_RC(upload(&h->mx_lam, llam.data(), llam.size()));
+  QD_RC(upload(&h->mx_unit, uni.data(), uni.size()));
+  QD_RC(upload(&h->mx_umat, upad.data(), upad.size()));
+  QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int nthreads = NL * rf_n1(n1);
+  int nb = 0, ncu = 0;
+  QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nthreads, lds));
+  QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
+  if (nb < 1) return QD_OK;
+  h->mx_nl = NL;
+  h->mx_gb = GB;
+  h->mx_maxu = std::max(maxu, 4);
+  h->mx_lds = lds;
+  h->mx_nbundle = (int)heads.size();
+  h->mx_slots = (long long)nb * std::max(1, ncu);
+  h->rp_mx = true;
+  return QD_OK;
+}
+
 int create_batch(qd_handle *h, const BatchSpec &S) {
   const qd_problem *p = S.base;
   const int n1 = p->r1.n, n2 = p->r2.n;
@@ -831,11 +860,17 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   const size_t elem = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   int gmax = 1;
   for (const Group &g : S.groups) gmax = std::max(gmax, (int)g.cols.size());
-  if (gmax > RP_GMAX) return fail(QD_ERR_ARG, "mixing groups larger than 32 (J > 31) are not supported");
+  // the FP32 mixing R pass with prefetch takes mixing groups of up to 48 lines
+  // on 64-, 128- and 256-point R grids; the other R passes up to 32
+  const bool mx_cand = h->prec == QD_PREC_MIXED && h->has_mix && (n1 == 64 || n1 == 128 || n1 == 256) && gmax <= 48;
+  if (gmax > RP_GMAX && !mx_cand)
+    return fail(QD_ERR_ARG, "mixing groups larger than 32 lines (J > 31) need mixed precision on a 64-, 128- or "
+                            "256-point R grid, and at most 48 lines (J <= 47)");
+  h->rp_legacy = gmax <= RP_GMAX;
   std::vector<int> bu0, ucols;
   std::vector<RUnit> units;
   const size_t tabsz = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float4);
-  {
+  if (h->rp_legacy) {
     // R pass: bundles of whole units; about 2048 elements per work group, at
     // least one mixing group, one line per thread slot.  Lengths with a
     // compile-time kernel use 8 or 32 lines per group.
@@ -916,7 +951,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   // register-FFT passes for the fused step (other flag sets keep the
   // Stockham kernels above)
   h->rp_rf = h->rl_rf = false;
-  if (rf_length(n1)) {
+  if (rf_length(n1) && h->rp_legacy) {
     const int n1f = rf_n1(n1);
     int lw = 256 / n1f;
     if (h->has_mix) lw = std::max(lw, h->rp_ml);  // a bundle holds up to rp_ml lines
@@ -1004,7 +1039,14 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_RC(upload(&u, S.umat.data(), S.umat.size()));
     h->umat = u;
   } else {
-    std::vector<float> uf(S.umat.begin(), S.umat.end());
+    std::vector<float> uf(S.umat.size());
+    for (const Group &g : S.groups) {
+      if (g.u0 < 0) continue;
+      const int gs = (int)g.cols.size();
+      std::vector<float> q;
+      qdt::round_orthogonal_columns(S.umat.data() + g.u0, gs, q);
+      std::copy(q.begin(), q.end(), uf.begin() + g.u0);
+    }
     float *u = nullptr;
     QD_RC(upload(&u, uf.data(), uf.size()));
     h->umat = u;
@@ -1141,6 +1183,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   QD_RC(put_table(h, &h->twR, twR));
   QD_RC(put_table(h, &h->twr, twr));
   QD_RC(upload(&h->fcol_lam, fcol_lam.data(), fcol_lam.size()));
+  if (mx_cand) QD_RC(setup_mx(h, S, fcol_lam, gmax));
   if (h->rp_rf && h->has_mix) {
     // per-bundle line tables of the mixing R pass: element offset and D_R row per line
     std::vector<long long> loff;
@@ -1385,7 +1428,8 @@ void release(qd_handle *h) {
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
-                  h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps};
+                  h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mx_head,  h->mx_off,
+                  h->mx_lam,   h->mx_unit, h->mx_umat};
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (void *b : bufs)
@@ -1575,7 +1619,30 @@ int autotune_handle(qd_handle *h) {
     *flag = t_rf <= t_st;
     return QD_OK;
   };
-  QD_RC(pick_pass(&h->rp_rf, true));
+  // R pass: the prefetching mixing kernel, the register FFT with LDS-staged
+  // mixing, or the Stockham kernels, whichever of them this handle has
+  {
+    const bool has_mx = h->rp_mx, has_rf = h->rp_rf, has_st = h->rp_legacy;
+    auto runR = [&] {
+      return f64 ? launch_rpass<double>(h, full, (double2 *)lpsi) : launch_rpass<float>(h, full, (float2 *)lpsi);
+    };
+    float best = 1e30f;
+    int pick = has_mx ? 0 : has_rf ? 1 : 2;
+    for (int c = 0; c < 3; c++) {
+      if ((c == 0 && !has_mx) || (c == 1 && !has_rf) || (c == 2 && !has_st)) continue;
+      if (c == 2 && !has_mx && !has_rf) continue;  // the only choice
+      h->rp_mx = c == 0;
+      h->rp_rf = c == 1;
+      float ms = 0;
+      QD_RC(time_launches(h, runR, reps, &ms));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    h->rp_mx = pick == 0;
+    h->rp_rf = pick == 1;
+  }
   QD_RC(pick_pass(&h->rl_rf, false));
   h->tuned = true;
   return QD_OK;
@@ -1628,6 +1695,8 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[5] = h->rp_lw;
   vals[6] = h->rp_ml;
   vals[7] = h->rl_lpb;
+  vals[8] = h->rp_mx ? h->mx_nl : 0;
+  vals[9] = h->rp_mx ? h->mx_gb : 0;
   return QD_OK;
 }
 

@@ -1,0 +1,224 @@
+// qd_dispatch.hip -- definitions of the getters declared in qd_dispatch.hpp.
+//
+// The file is compiled once per part (-DQD_PART=n), each part instantiating
+// one family of kernels, so that the template instantiations -- the bulk of
+// the build -- compile in parallel:
+//   0  Stockham passes, FP32      1  Stockham passes, FP64
+//   2  register-FFT passes, FP32  3  register-FFT passes, FP64 (no mixing)
+//   4  mixed theta GEMM
+//   5  mixing R pass with prefetch, 256-point lines
+//   6  mixing R pass with prefetch, 64- and 128-point lines
+//   7  register-FFT R pass with LDS-staged mixing, FP32
+//   8  register-FFT R pass with LDS-staged mixing, FP64
+// Every kernel instantiation lives in exactly one part.
+#include <hip/hip_runtime.h>
+
+#include "qd_dispatch.hpp"
+#include "qd_fft.hpp"
+#include "qd_kernels.hpp"
+#include "qd_rfft.hpp"
+
+#ifndef QD_PART
+#error "compile with -DQD_PART=<0..8>"
+#endif
+
+namespace qd {
+
+#define QD_CT_LENGTHS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+#if QD_PART == 0
+bool ct_length(int n) {
+  switch (n) {
+#define QD_CASE(N) case N:
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+#endif
+
+#if QD_PART == 0 || QD_PART == 1
+template <typename T, int N, int ML>
+const void *rpass_ct_fn(bool mix) {
+  if constexpr (256 / ML * FFT_BPT >= ct_maxnb(N))
+    return mix ? (const void *)k_rpass_ct<T, N, ML, true> : (const void *)k_rpass_ct<T, N, ML, false>;
+  else return nullptr;
+}
+template <typename T, int N, int LPB>
+const void *rline_ct_fn() {
+  if constexpr (256 / LPB * FFT_BPT >= ct_maxnb(N)) return (const void *)k_rline_ct<T, N, LPB>;
+  else return nullptr;
+}
+// kernel for (length, lines per group); nullptr -> generic kernel
+template <typename T>
+const void *rpass_kernel(int n, int ml, bool mix) {
+  switch (n) {
+#define QD_CASE(N) \
+  case N:          \
+    return ml == 8 ? rpass_ct_fn<T, N, 8>(mix) : ml == 32 ? rpass_ct_fn<T, N, 32>(mix) : nullptr;
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+template <typename T>
+const void *rline_kernel(int n, int lpb) {
+  switch (n) {
+#define QD_CASE(N) \
+  case N:          \
+    return lpb == 16 ? rline_ct_fn<T, N, 16>() : lpb == 8 ? rline_ct_fn<T, N, 8>() : nullptr;
+    QD_CT_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+
+#if QD_PART == 0
+template const void *rpass_kernel<float>(int, int, bool);
+template const void *rline_kernel<float>(int, int);
+#else
+template const void *rpass_kernel<double>(int, int, bool);
+template const void *rline_kernel<double>(int, int);
+#endif
+#endif  // Stockham parts
+
+#define QD_RF_LENGTHS(X) X(24, 8) X(32, 8) X(48, 8) X(64, 8) X(96, 16) X(128, 16) X(192, 16) X(256, 16) X(384, 32) X(512, 32)
+
+#if QD_PART == 2 || QD_PART == 3
+// register-FFT passes: lengths with a factorisation N1 x N2 (rf_n1)
+template <typename T, int N1, int N2>
+const void *rpass_rf_fn(int lw, size_t *lds) {
+  constexpr int LW0 = 256 / N1;
+  if (lw != LW0) return nullptr;
+  *lds = rpass_rf_lds<T, N1, N2, false, LW0>();
+  return (const void *)k_rpass_rf<T, N1, N2, false, LW0>;
+}
+template <typename T>
+const void *rpass_rf_kernel(int n, bool mix, int lw, size_t *lds) {
+  if (mix) return rpass_rf_mix_kernel<T>(n, lw, lds);
+  switch (n) {
+#define QD_CASE(N, N1) \
+  case N:              \
+    return rpass_rf_fn<T, N1, N / N1>(lw, lds);
+    QD_RF_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+template <typename T>
+const void *rline_rf_kernel(int n, size_t *lds, int *lpb) {
+  switch (n) {
+#define QD_CASE(N, N1)                             \
+  case N:                                          \
+    *lds = rline_rf_lds<T, N1, N / N1>();          \
+    *lpb = 256 / N1;                               \
+    return (const void *)k_rline_rf<T, N1, N / N1>;
+    QD_RF_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+#if QD_PART == 2
+template const void *rpass_rf_kernel<float>(int, bool, int, size_t *);
+template const void *rline_rf_kernel<float>(int, size_t *, int *);
+#else
+template const void *rpass_rf_kernel<double>(int, bool, int, size_t *);
+template const void *rline_rf_kernel<double>(int, size_t *, int *);
+#endif
+#endif  // register-FFT parts
+
+#if QD_PART == 7 || QD_PART == 8
+// register-FFT R pass with K <-> l mixing staged in LDS (lw lines per work group)
+template <typename T, int N1, int N2>
+const void *rpass_rf_mix_fn(int lw, size_t *lds) {
+  constexpr int LW0 = 256 / N1;
+  if (lw == LW0) {
+    *lds = rpass_rf_lds<T, N1, N2, true, LW0>();
+    return (const void *)k_rpass_rf<T, N1, N2, true, LW0>;
+  }
+  if (lw == 2 * LW0) {  // 512 threads: mixing groups of up to 2 * LW0 lines
+    *lds = rpass_rf_lds<T, N1, N2, true, 2 * LW0>();
+    return (const void *)k_rpass_rf<T, N1, N2, true, 2 * LW0>;
+  }
+  return nullptr;
+}
+template <typename T>
+const void *rpass_rf_mix_kernel(int n, int lw, size_t *lds) {
+  switch (n) {
+#define QD_CASE(N, N1) \
+  case N:              \
+    return rpass_rf_mix_fn<T, N1, N / N1>(lw, lds);
+    QD_RF_LENGTHS(QD_CASE)
+#undef QD_CASE
+    default:
+      return nullptr;
+  }
+}
+#if QD_PART == 7
+template const void *rpass_rf_mix_kernel<float>(int, int, size_t *);
+#else
+template const void *rpass_rf_mix_kernel<double>(int, int, size_t *);
+#endif
+#endif  // register-FFT mixing parts
+
+#if QD_PART == 4
+// matrix-stationary mixed theta kernel for KS k-steps (KP = NP = 32 KS)
+const void *theta_mx_kernel(int ks, int pro, int epi) {
+#define QD_TMX(KS_)                                                                    \
+  if (ks == KS_) {                                                                     \
+    if (pro == 0 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 0>); \
+    if (pro == 1 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 1, 0>); \
+    if (pro == 2 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 2, 0>); \
+    if (pro == 0 && epi == 1) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 1>); \
+  }
+  QD_TMX(1)
+  QD_TMX(2)
+  QD_TMX(3)
+  QD_TMX(4)
+#undef QD_TMX
+  return nullptr;
+}
+
+#endif  // theta part
+
+#if QD_PART == 5 || QD_PART == 6
+template <int N1, int N2, int NL, int GB>
+const void *rpass_mx_fn(size_t *lds) {
+  *lds = rpass_mx_lds_fixed<N1, N2, NL>();
+  return (const void *)k_rpass_mx<N1, N2, NL, GB>;
+}
+template <int N1, int N2>
+const void *rpass_mx_len(int nl, int gb, size_t *lds) {
+  if (nl == 32) {
+    if (gb == 8) return rpass_mx_fn<N1, N2, 32, 8>(lds);
+    if (gb == 16) return rpass_mx_fn<N1, N2, 32, 16>(lds);
+    if (gb == 24) return rpass_mx_fn<N1, N2, 32, 24>(lds);
+    if (gb == 32) return rpass_mx_fn<N1, N2, 32, 32>(lds);
+  } else if (nl == 48 && gb == 48) {
+    return rpass_mx_fn<N1, N2, 48, 48>(lds);
+  }
+  return nullptr;
+}
+#endif
+#if QD_PART == 5
+const void *rpass_mx_kernel(int n, int nl, int gb, size_t *lds_fixed) {
+  if (n == 256) return rpass_mx_len<16, 16>(nl, gb, lds_fixed);
+  return rpass_mx_kernel_small(n, nl, gb, lds_fixed);
+}
+#endif
+#if QD_PART == 6
+const void *rpass_mx_kernel_small(int n, int nl, int gb, size_t *lds_fixed) {
+  if (n == 64) return rpass_mx_len<8, 8>(nl, gb, lds_fixed);
+  if (n == 128) return rpass_mx_len<16, 8>(nl, gb, lds_fixed);
+  return nullptr;
+}
+#endif
+
+}  // namespace qd

@@ -117,7 +117,7 @@ struct BlockDesc {
 // propagator (timeEvol.C:519-538).  sqrt_ref is the reference's pow(c, 0.5)
 // with arg() = asin(Im/|c|) plus quadrant fixes (complex.h:60-68,148-159); it
 // equals exp(+i V dt/2) only while |V dt| < pi, which is reproduced here.
-__global__ void k_prephase_cpu(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
+static __global__ void k_prephase_cpu(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
                                double dt, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
@@ -143,7 +143,7 @@ __global__ void k_prephase_cpu(double2 *psi, const double *V, const long long *d
 
 // psi *= exp(+i V dt/2): the CUDA propagator's first-step phase
 // (evolutionCUDA2.cu:507, evolution_with_potential(-dt/2)).
-__global__ void k_prephase_cuda(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
+static __global__ void k_prephase_cuda(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
                                 double dt, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
@@ -212,11 +212,11 @@ __global__ void k_kin2d(typename cx_of<T>::type *lpsi, const typename tab_of<T>:
   lpsi[x] = tmul(tmul(lpsi[x], kinR[i], false), kinr[j], false);
 }
 
-__global__ void k_f64_to_f32(const double2 *a, float2 *b, size_t n) {
+static __global__ void k_f64_to_f32(const double2 *a, float2 *b, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x < n) b[x] = make_float2((float)a[x].x, (float)a[x].y);
 }
-__global__ void k_f32_to_f64(const float2 *a, double2 *b, size_t n) {
+static __global__ void k_f32_to_f64(const float2 *a, double2 *b, size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x < n) b[x] = make_double2((double)a[x].x, (double)a[x].y);
 }
@@ -1158,11 +1158,11 @@ constexpr int FX_TC = 16;
 
 // counter += add (one thread): advances device-side step counts between
 // kernels, so a captured graph can be replayed without new arguments
-__global__ void k_counter_add(long long *counter, int add) {
+static __global__ void k_counter_add(long long *counter, int add) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter += add;
 }
 
-__global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restrict__ bpsi,
+static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restrict__ bpsi,
                                                          const double2 *__restrict__ bdpsi,
                                                          double2 *__restrict__ fai, double2 *__restrict__ dfai,
                                                          const double *__restrict__ energies, int nx, int ne,
@@ -1234,7 +1234,7 @@ __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restri
 // P(E) of block b: Im(sum_k w_k sum_i conj(fai) dfai) / eta^2 * dr1/mu2
 // (CRPAux.F:34-66); sign = -1 reproduces the CUDA propagator's
 // Zdotc(dfai, fai) (evolutionCUDA2.cu:972-977).  Grid (ne, nblocks).
-__global__ __launch_bounds__(256) void k_crp(const double2 *__restrict__ fai, const double2 *__restrict__ dfai,
+static __global__ __launch_bounds__(256) void k_crp(const double2 *__restrict__ fai, const double2 *__restrict__ dfai,
                                              const double *__restrict__ dcol_w, const double *__restrict__ eta_sq,
                                              const BlockDesc *__restrict__ blocks, double *__restrict__ crp,
                                              int n1, int nx, int ne, double dr1_mu2, double sign) {

@@ -461,6 +461,251 @@ constexpr size_t rpass_rf_lds() {
                : (size_t)2 * N * sizeof(typename tab_of<T>::type));
 }
 
+// ---- mixing R pass with prefetch (FP32) ------------------------------------
+// Bundles of whole mixing units (the K lines of one (J, j, r)), at most NL
+// lines and two distinct unit matrices each, with host-built fixed-stride
+// records.  A work group of NL * N1 threads stages one bundle in LDS, one line
+// per row; the row doubles as that line's FFT exchange buffer.  Per bundle:
+//   mix K -> l      one (unit, point) per thread, the unit's g inputs in
+//                   registers, outputs written in place (only this thread
+//                   touches this point of the unit's lines);
+//   FFT phase       line L: D_R/2, forward FFT, kinetic, inverse FFT, D_R/2;
+//   mix l -> K      read the g values of the point, store the K lines
+//                   straight to HBM.
+// The next bundle's descriptors are loaded during the K -> l mixing and its
+// lines and matrices during the FFT and the l -> K mixing, so the HBM traffic
+// of one bundle overlaps the arithmetic of the previous one.
+struct MxHead {
+  int nl, nu;    // lines and units of the bundle
+  int msrc[2];   // offsets in umat of its matrices (U, then U^T at + g4^2)
+  int mg4[2];    // their padded orders (multiples of 4; 0 = unused)
+  int pad[2];
+};
+
+struct RMixArgs {
+  float2 *data;
+  const float4 *tw;          // exp(-2 pi i e / n1), hi + lo
+  const float4 *kin;         // exp(-i T1 dt) / n1
+  const float4 *dR;          // D_R/2 rows [lam][n1]
+  const MxHead *head;        // [nbundle]
+  const long long *line_off;  // [nbundle][NL] element offset of each line
+  const int *line_lam;       // [nbundle][NL] D_R row of each line
+  const int *unit;           // [nbundle][NL] line0 | g << 8 | slot << 16 (slot 0: no mixing)
+  const float *umat;         // padded U and U^T of every unit matrix
+  int nbundle;
+  int maxu;                  // LDS entries per direction for the bundle's matrices
+};
+
+template <int N1, int N2, int NL>
+constexpr int mx_row_stride() {
+  constexpr int N = N1 * N2, LX = N1 * (N2 + 1), SPA = N + N / 32 + 1;
+  return LX > SPA ? LX : SPA;
+}
+template <int N1, int N2, int NL>
+constexpr size_t rpass_mx_lds_fixed() {  // without the unit matrices (2 * maxu floats)
+  return rf_table_bytes<N1, N2>(sizeof(float4)) + (size_t)NL * mx_row_stride<N1, N2, NL>() * sizeof(float2);
+}
+
+template <int N1, int N2, int NL, int GB>
+__global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
+  typedef float2 C;
+  typedef float4 TB;
+  constexpr int N = N1 * N2, NT = NL * N1, XS = N2 + 1;
+  constexpr int SP = mx_row_stride<N1, N2, NL>();
+  constexpr int UPR = NT / N;                        // units per mixing round
+  constexpr int UMAX = 2 * GB * GB;                  // bound on the matrix entries of a bundle
+  constexpr int UPT = (UMAX + NT - 1) / NT;          // prefetched matrix entries per thread and direction
+  static_assert(N % 64 == 0 && NT % N == 0, "the points of a unit must fill whole waves");
+  static_assert(GB % 4 == 0 && GB <= NL && NL <= 64, "unit size bucket");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
+  TB *s_kin = s_twa + N1 * XS;
+  C *stg = reinterpret_cast<C *>(s_kin + N);          // [NL][SP]
+  float *s_U = reinterpret_cast<float *>(stg + NL * SP);
+  float *s_UT = s_U + A.maxu;
+  __shared__ long long s_off[NL];
+  __shared__ int s_unit[NL];
+  const int t = threadIdx.x, a = t % N1, L = t / N1;
+  rf_tables<float, N1, N2>(s_twa, s_kin, A.tw, A.kin);
+
+  // descriptors of a bundle (registers): header, this thread's line, unit t
+  int4 h0, h1;
+  long long d_off = 0;
+  int d_lam = 0, d_unit = 0;
+  auto fetch_desc = [&](int bb, bool has) {
+    if (has) {
+      const int4 *hp = reinterpret_cast<const int4 *>(A.head + bb);
+      h0 = hp[0];
+      h1 = hp[1];
+      d_off = A.line_off[(size_t)bb * NL + L];
+      d_lam = A.line_lam[(size_t)bb * NL + L];
+      if (t < NL) d_unit = A.unit[(size_t)bb * NL + t];
+    } else {
+      h0 = make_int4(0, 0, 0, 0);
+      h1 = make_int4(0, 0, 0, 0);
+    }
+  };
+  // lines and matrices of a bundle (registers), from its descriptors
+  C xn[N2];
+  float un[UPT], unt[UPT];
+  auto fetch_data = [&]() {
+    const int nl = h0.x;
+    if (L < nl) {
+      const C *src = A.data + d_off;
+#pragma unroll
+      for (int m = 0; m < N2; m++) xn[m] = src[a + N1 * m];
+    }
+    const int g0 = h1.x, g1 = h1.y;     // mg4[0], mg4[1]
+    const int n0 = g0 * g0, nu_ = n0 + g1 * g1;
+#pragma unroll
+    for (int q = 0; q < UPT; q++) {
+      const int i = t + q * NT;
+      if (i < nu_) {
+        const int slot = i >= n0, ii = slot ? i - n0 : i, gg = slot ? g1 : g0;
+        const float *src = A.umat + (slot ? h0.w : h0.z);
+        un[q] = src[ii];
+        unt[q] = src[gg * gg + ii];
+      }
+    }
+  };
+
+  int b = blockIdx.x;
+  if (b < A.nbundle) {
+    fetch_desc(b, true);
+    fetch_data();
+  }
+  for (; b < A.nbundle; b += gridDim.x) {
+    const int nl = __builtin_amdgcn_readfirstlane(h0.x), nu = __builtin_amdgcn_readfirstlane(h0.y);
+    const int g40 = __builtin_amdgcn_readfirstlane(h1.x), g41 = __builtin_amdgcn_readfirstlane(h1.y);
+    const int lam = d_lam;
+    __syncthreads();  // the previous bundle is done with LDS (first time: tables)
+    if (L < nl) {
+#pragma unroll
+      for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = xn[m];
+      if (a == 0) s_off[L] = d_off;
+    }
+    if (t < NL) s_unit[t] = d_unit;
+    {
+      const int nuu = g40 * g40 + g41 * g41;
+#pragma unroll
+      for (int q = 0; q < UPT; q++) {
+        const int i = t + q * NT;
+        if (i < nuu) {
+          s_U[i] = un[q];
+          s_UT[i] = unt[q];
+        }
+      }
+    }
+    const int bn = b + (int)gridDim.x;
+    fetch_desc(bn, bn < A.nbundle);
+    __syncthreads();
+    // K -> l: c_m = sum_K U[K][m] c_K  (rows of U^T)
+    for (int u0 = 0; u0 < nu; u0 += UPR) {
+      const int u = u0 + t / N, p = t % N;
+      if (u < nu) {
+        const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
+        const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
+        if (slot) {
+          const int g4 = slot == 1 ? g40 : g41;
+          const float *UT = s_UT + (slot == 1 ? 0 : g40 * g40);
+          C *col = stg + line0 * SP + p;
+          C x[GB];
+#pragma unroll
+          for (int K = 0; K < GB; K++) {
+            if (K < g) x[K] = col[K * SP];
+            else x[K] = make_float2(0.f, 0.f);
+          }
+#pragma unroll
+          for (int m = 0; m < GB; m++) {
+            if (m < g) {
+              const float *row = UT + m * g4;
+              float ax = 0.f, ay = 0.f;
+#pragma unroll
+              for (int k4 = 0; k4 < GB / 4; k4++) {
+                if (4 * k4 < g) {
+                  const float4 w = *reinterpret_cast<const float4 *>(row + 4 * k4);
+                  ax = fmaf(x[4 * k4].x, w.x, ax);
+                  ay = fmaf(x[4 * k4].y, w.x, ay);
+                  ax = fmaf(x[4 * k4 + 1].x, w.y, ax);
+                  ay = fmaf(x[4 * k4 + 1].y, w.y, ay);
+                  ax = fmaf(x[4 * k4 + 2].x, w.z, ax);
+                  ay = fmaf(x[4 * k4 + 2].y, w.z, ay);
+                  ax = fmaf(x[4 * k4 + 3].x, w.w, ax);
+                  ay = fmaf(x[4 * k4 + 3].y, w.w, ay);
+                }
+              }
+              col[m * SP] = make_float2(ax, ay);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // the next bundle's lines and matrices stream in during the rest
+    fetch_data();
+    // FFT phase: line L in the l basis
+    {
+      const bool act = L < nl;
+      C *row = stg + L * SP;
+      const TB *ph = A.dR + (size_t)(act ? lam : 0) * N;
+      C v[N2];
+      if (act) {
+#pragma unroll
+        for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], ph[a + N1 * m], false);
+      }
+      rf_line<float, N1, N2>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
+      if (act) {
+#pragma unroll
+        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+      }
+    }
+    __syncthreads();
+    // l -> K and store: c_K = sum_m U[K][m] c_m  (rows of U)
+    for (int u0 = 0; u0 < nu; u0 += UPR) {
+      const int u = u0 + t / N, p = t % N;
+      if (u < nu) {
+        const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
+        const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
+        const C *col = stg + line0 * SP + p;
+        if (slot) {
+          const int g4 = slot == 1 ? g40 : g41;
+          const float *U = s_U + (slot == 1 ? 0 : g40 * g40);
+          C x[GB];
+#pragma unroll
+          for (int m = 0; m < GB; m++) {
+            if (m < g) x[m] = col[m * SP];
+            else x[m] = make_float2(0.f, 0.f);
+          }
+#pragma unroll
+          for (int K = 0; K < GB; K++) {
+            if (K < g) {
+              const float *row = U + K * g4;
+              float ax = 0.f, ay = 0.f;
+#pragma unroll
+              for (int m4 = 0; m4 < GB / 4; m4++) {
+                if (4 * m4 < g) {
+                  const float4 w = *reinterpret_cast<const float4 *>(row + 4 * m4);
+                  ax = fmaf(x[4 * m4].x, w.x, ax);
+                  ay = fmaf(x[4 * m4].y, w.x, ay);
+                  ax = fmaf(x[4 * m4 + 1].x, w.y, ax);
+                  ay = fmaf(x[4 * m4 + 1].y, w.y, ay);
+                  ax = fmaf(x[4 * m4 + 2].x, w.z, ax);
+                  ay = fmaf(x[4 * m4 + 2].y, w.z, ay);
+                  ax = fmaf(x[4 * m4 + 3].x, w.w, ax);
+                  ay = fmaf(x[4 * m4 + 3].y, w.w, ay);
+                }
+              }
+              A.data[s_off[line0 + K] + p] = make_float2(ax, ay);
+            }
+          }
+        } else {
+          for (int K = 0; K < g; K++) A.data[s_off[line0 + K] + p] = col[K * SP];
+        }
+      }
+    }
+  }
+}
+
 // r pass: lines (column, i) of N = n2 points at stride n1; a batch is 256 / N1
 // consecutive i of one column (n1 a multiple of 256 / N1), so the lanes of a
 // load read consecutive addresses.  Persistent over batches, with the next

@@ -193,4 +193,43 @@ inline void coriolis_eigen(int J, int j, std::vector<double> &U, std::vector<int
   }
 }
 
+// FP32 copy of an orthogonal g x g matrix U (row K, column m), rounded so that
+// every column keeps unit length to second order.  The mixing applies U^T and
+// then U to the same point every step; with plain rounding, U_f = U + E, the
+// composite's norm changes by 4 sum_m (U^T E)_mm |z_m|^2 per step, coherently
+// over thousands of steps.  Here each column's entries are moved between
+// their two nearest floats (at most 2 ulp from the value) until
+// sum_K U[K][m] E[K][m] is as small as the choices allow.
+inline void round_orthogonal_columns(const double *U, int g, std::vector<float> &Q) {
+  Q.resize((size_t)g * g);
+  for (size_t i = 0; i < Q.size(); i++) Q[i] = (float)U[i];
+  for (int m = 0; m < g; m++) {
+    double s = 0;
+    for (int K = 0; K < g; K++) s += U[(size_t)K * g + m] * ((double)Q[(size_t)K * g + m] - U[(size_t)K * g + m]);
+    for (int it = 0; it < 4 * g; it++) {
+      double best = std::fabs(s);
+      int bk = -1;
+      float bv = 0;
+      for (int K = 0; K < g; K++) {
+        const double u = U[(size_t)K * g + m];
+        const float q = Q[(size_t)K * g + m];
+        const double lim = 2.0 * std::fabs((double)std::nextafter((float)u, INFINITY) - (double)(float)u);
+        for (float v : {std::nextafter(q, INFINITY), std::nextafter(q, -INFINITY)}) {
+          if (std::fabs((double)v - u) > lim) continue;
+          const double s2 = s + u * ((double)v - (double)q);
+          if (std::fabs(s2) < best) {
+            best = std::fabs(s2);
+            bk = K;
+            bv = v;
+          }
+        }
+      }
+      if (bk < 0) break;
+      const double u = U[(size_t)bk * g + m];
+      s += u * ((double)bv - (double)Q[(size_t)bk * g + m]);
+      Q[(size_t)bk * g + m] = bv;
+    }
+  }
+}
+
 }  // namespace qdt

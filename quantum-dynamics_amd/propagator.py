@@ -170,10 +170,11 @@ class Propagator:
 
     def tuning(self) -> dict:
         """Kernel configuration in use (qd_get_tuning)."""
-        v = (C.c_int32 * 8)()
+        v = (C.c_int32 * 10)()
         _abi.check(self.lib.qd_get_tuning(self.h, v))
         keys = ("autotuned", "theta_ksteps", "theta_tiles_per_group", "rpass_register_fft",
-                "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group")
+                "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group",
+                "rpass_prefetch_mix_bundle_lines", "rpass_prefetch_mix_group_bucket")
         return dict(zip(keys, list(v)))
 
     def timing(self, which: int) -> float:

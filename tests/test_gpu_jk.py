@@ -78,3 +78,54 @@ def test_jk_norm_conserved_without_absorbers(qd, jkcase, prec):
         prop.propagate(60)
         n1 = prop.norm()
     assert abs(n1 - n0) < (1e-12 if prec == "fp64" else 1e-6) * n0
+
+
+def test_jk_prefetch_mixing_mixed_within_1e4_of_fp64(qd):
+    """Mixed-precision batch on a 64-point R grid (prefetching FP32 mixing R
+    pass) against the FP64 batch over 1500 steps: per-energy |dP^J(E)| within
+    1e-4, and the norm of a no-absorber run within 1e-5 of its start."""
+    pb = qd.problem
+    base = pb.config(0, nR=64, nr=32, ntheta=12, steps=1500, e0_ev=0.6)
+    jk = pb.make_jk_problem(base, pb.H_H2, [0, 4, 9], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
+    print("case: jk mixed vs fp64, prefetch mixing, 1500 steps", flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with qd.propagator.JKPropagator(jk, precision=prec) as prop:
+            if prec == "mixed":
+                assert prop.tuning()["rpass_prefetch_mix_bundle_lines"] == 32
+            prop.propagate(base.total_steps)
+            out[prec] = prop.crp_by_J()
+    worst = max(float(np.max(np.abs(out["mixed"][J] - out["fp64"][J]))) for J in out["fp64"])
+    print("max |dP^J(E)| mixed vs fp64: %.2e" % worst)
+    assert max(float(np.max(out["fp64"][J])) for J in out["fp64"]) > 1e-3  # a non-trivial P
+    assert worst < 1e-4
+
+    b2 = base.copy()
+    b2.dump1 = None
+    b2.dump2 = None
+    jk2 = pb.make_jk_problem(b2, pb.H_H2, [0, 4, 9], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
+    import jk_oracle
+    n0 = jk_oracle.norm(jk2, jk2.psi)
+    with qd.propagator.JKPropagator(jk2, precision="mixed") as prop:
+        prop.propagate(1500)
+        n1 = prop.norm()
+    print("norm drift over 1500 steps: %.2e" % ((n1 - n0) / n0))
+    assert abs(n1 - n0) < 1e-5 * n0
+
+
+def test_jk_groups_above_32_lines_match_oracle(qd):
+    """J = 33: 34 Coriolis-coupled K blocks per (J, j >= 33), bundled 48 lines
+    at a time by the prefetching mixing R pass (mixed precision only), against
+    the FP64 numpy oracle."""
+    import jk_oracle
+
+    pb = qd.problem
+    base = pb.config(0, nR=64, nr=16, ntheta=8, steps=12, e0_ev=0.6)
+    jk = pb.make_jk_problem(base, pb.H_H2, [33], jmax=35, R0=8.0, delta=0.3, e0_ev=0.6)
+    print("case: jk J=33 groups of 34 lines, mixed vs oracle", flush=True)
+    psi_o, PJ, _ = jk_oracle.propagate(jk, 12)
+    with qd.propagator.JKPropagator(jk, precision="mixed") as prop:
+        assert prop.tuning()["rpass_prefetch_mix_bundle_lines"] == 48
+        prop.propagate(12)
+        psi_g = prop.psi().view(np.complex128)
+    assert rel(psi_g, psi_o) < 2e-5

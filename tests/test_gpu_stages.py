@@ -115,15 +115,26 @@ def test_fused_passes_with_phases(qd, small, prec):
         assert relerr(got, ref) < TOL[prec]
 
 
+# (R points, r points, J values, j_max): LDS-staged mixing (32 points), the
+# prefetching FP32 kernel with 32-line bundles (64 points, groups up to 10
+# lines) and with 48-line bundles (groups of 34 lines)
+MIX_CASES = {"staged": (32, 24, [0, 2, 3], 5), "prefetch32": (64, 16, [0, 3, 9], 11),
+             "prefetch48": (64, 16, [33], 35)}
+
+
+@pytest.mark.parametrize("case", sorted(MIX_CASES))
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_r_pass_mixing_jk(qd, prec):
+def test_r_pass_mixing_jk(qd, prec, case):
     """J > 0: the R pass mixes the K blocks of each (J, j) into the l basis,
     applies D_R/2 with l(l+1), transforms, and mixes back."""
+    nR, nr, Js, jmax = MIX_CASES[case]
+    if prec == "fp64" and max(Js) > 31:
+        pytest.skip("mixing groups above 32 lines run in mixed precision only")
     pb = qd.problem
-    base = pb.config(0, nR=32, nr=24, ntheta=6, steps=2)
-    jk = pb.make_jk_problem(base, pb.H_H2, [0, 2, 3], jmax=5)
+    base = pb.config(0, nR=nR, nr=nr, ntheta=6, steps=2)
+    jk = pb.make_jk_problem(base, pb.H_H2, Js, jmax=jmax)
     n1, n2, dt = base.n1, base.n2, base.dt
-    print("case: R pass mixing", prec, flush=True)
+    print("case: R pass mixing", prec, case, flush=True)
     I1 = 1 / (2 * base.mass1 * base.r1 ** 2)
     T1 = momenta(n1, n1 * base.dr1) ** 2 / (2 * base.mass1)
     kin = np.exp(-1j * dt * T1)
@@ -133,6 +144,8 @@ def test_r_pass_mixing_jk(qd, prec):
         prop.set_fbr(as_f64(y))
         prop.debug_stage(1)
         got = prop.get_fbr().view(np.complex128)
+        uses_prefetch = prop.tuning()["rpass_prefetch_mix_bundle_lines"] > 0
+    assert uses_prefetch == (prec == "mixed" and case.startswith("prefetch"))
     # numpy statement
     ref = y.copy()
     cols = {}
