@@ -466,9 +466,9 @@ constexpr size_t rpass_rf_lds() {
 // lines and two distinct unit matrices each, with host-built fixed-stride
 // records.  A work group of NL * N1 threads stages one bundle in LDS, one line
 // per row; the row doubles as that line's FFT exchange buffer.  Per bundle:
-//   mix K -> l      one (unit, point) per thread, the unit's g inputs in
-//                   registers, outputs written in place (only this thread
-//                   touches this point of the unit's lines);
+//   mix K -> l      one (unit, point) per thread, the unit's g outputs
+//                   accumulated in registers and written in place (only this
+//                   thread touches this point of the unit's lines);
 //   FFT phase       line L: D_R/2, forward FFT, kinetic, inverse FFT, D_R/2;
 //   mix l -> K      read the g values of the point, store the K lines
 //                   straight to HBM.
@@ -506,13 +506,65 @@ constexpr size_t rpass_mx_lds_fixed() {  // without the unit matrices (2 * maxu 
   return rf_table_bytes<N1, N2>(sizeof(float4)) + (size_t)NL * mx_row_stride<N1, N2, NL>() * sizeof(float2);
 }
 
+// acc_n = sum_k M[k][n] x_k over the G x G matrix M (row stride G, zero
+// padded beyond the unit's g lines) for the lines line0 .. line0 + g - 1 of
+// the staging rows at point p; out(n, acc_n) for n < g.  The loops have no
+// branches, so the compiler issues LDS reads ahead of the FMAs.
+template <int G, int SP, int NL, typename OUT>
+__device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, int p, const float *M, OUT out) {
+  float2 acc[G];
+#pragma unroll
+  for (int n = 0; n < G; n++) acc[n] = make_float2(0.f, 0.f);
+  // two k-steps per iteration: enough LDS reads in flight to cover their
+  // latency without holding the whole matrix in registers
+#pragma unroll 2
+  for (int k = 0; k < G; k++) {
+    const float2 xr = stg[min(line0 + k, NL - 1) * SP + p];
+    const float2 x = make_float2(k < g ? xr.x : 0.f, k < g ? xr.y : 0.f);
+#pragma unroll
+    for (int n4 = 0; n4 < G / 4; n4++) {
+      const float4 w = *reinterpret_cast<const float4 *>(M + k * G + 4 * n4);
+      acc[4 * n4].x = fmaf(x.x, w.x, acc[4 * n4].x);
+      acc[4 * n4].y = fmaf(x.y, w.x, acc[4 * n4].y);
+      acc[4 * n4 + 1].x = fmaf(x.x, w.y, acc[4 * n4 + 1].x);
+      acc[4 * n4 + 1].y = fmaf(x.y, w.y, acc[4 * n4 + 1].y);
+      acc[4 * n4 + 2].x = fmaf(x.x, w.z, acc[4 * n4 + 2].x);
+      acc[4 * n4 + 2].y = fmaf(x.y, w.z, acc[4 * n4 + 2].y);
+      acc[4 * n4 + 3].x = fmaf(x.x, w.w, acc[4 * n4 + 3].x);
+      acc[4 * n4 + 3].y = fmaf(x.y, w.w, acc[4 * n4 + 3].y);
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < G; n++)
+    if (n < g) out(n, acc[n]);
+}
+// mx_apply for the unit's padded order 4 ceil(g / 4) (at most GB)
+template <int GB, int SP, int NL, typename OUT>
+__device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int p, const float *M, OUT out) {
+#define QD_MXA(G_) mx_apply<G_, SP, NL>(stg, line0, g, p, M, out)
+  switch ((g + 3) >> 2) {
+    case 1: QD_MXA(4); break;
+    case 2: if constexpr (GB >= 8) QD_MXA(8); break;
+    case 3: if constexpr (GB >= 12) QD_MXA(12); break;
+    case 4: if constexpr (GB >= 16) QD_MXA(16); break;
+    case 5: if constexpr (GB >= 20) QD_MXA(20); break;
+    case 6: if constexpr (GB >= 24) QD_MXA(24); break;
+    case 7: if constexpr (GB >= 28) QD_MXA(28); break;
+    case 8: if constexpr (GB >= 32) QD_MXA(32); break;
+    case 9: if constexpr (GB >= 36) QD_MXA(36); break;
+    case 10: if constexpr (GB >= 40) QD_MXA(40); break;
+    case 11: if constexpr (GB >= 44) QD_MXA(44); break;
+    default: if constexpr (GB >= 48) QD_MXA(48); break;
+  }
+#undef QD_MXA
+}
+
 template <int N1, int N2, int NL, int GB>
 __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
   typedef float2 C;
   typedef float4 TB;
   constexpr int N = N1 * N2, NT = NL * N1, XS = N2 + 1;
   constexpr int SP = mx_row_stride<N1, N2, NL>();
-  constexpr int UPR = NT / N;                        // units per mixing round
   constexpr int UMAX = 2 * GB * GB;                  // bound on the matrix entries of a bundle
   constexpr int UPT = (UMAX + NT - 1) / NT;          // prefetched matrix entries per thread and direction
   static_assert(N % 64 == 0 && NT % N == 0, "the points of a unit must fill whole waves");
@@ -599,44 +651,17 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
     const int bn = b + (int)gridDim.x;
     fetch_desc(bn, bn < A.nbundle);
     __syncthreads();
-    // K -> l: c_m = sum_K U[K][m] c_K  (rows of U^T)
-    for (int u0 = 0; u0 < nu; u0 += UPR) {
-      const int u = u0 + t / N, p = t % N;
-      if (u < nu) {
+    // K -> l: c_m = sum_K U[K][m] c_K, one (unit, point) per thread, in place
+    // (only this thread touches this point of the unit's lines)
+    for (int i0 = 0; i0 < nu * N; i0 += NT) {
+      const int i = i0 + t;
+      if (i < nu * N) {  // uniform per wave (N is a multiple of 64)
+        const int u = i / N, p = i % N;
         const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
         const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
         if (slot) {
-          const int g4 = slot == 1 ? g40 : g41;
-          const float *UT = s_UT + (slot == 1 ? 0 : g40 * g40);
-          C *col = stg + line0 * SP + p;
-          C x[GB];
-#pragma unroll
-          for (int K = 0; K < GB; K++) {
-            if (K < g) x[K] = col[K * SP];
-            else x[K] = make_float2(0.f, 0.f);
-          }
-#pragma unroll
-          for (int m = 0; m < GB; m++) {
-            if (m < g) {
-              const float *row = UT + m * g4;
-              float ax = 0.f, ay = 0.f;
-#pragma unroll
-              for (int k4 = 0; k4 < GB / 4; k4++) {
-                if (4 * k4 < g) {
-                  const float4 w = *reinterpret_cast<const float4 *>(row + 4 * k4);
-                  ax = fmaf(x[4 * k4].x, w.x, ax);
-                  ay = fmaf(x[4 * k4].y, w.x, ay);
-                  ax = fmaf(x[4 * k4 + 1].x, w.y, ax);
-                  ay = fmaf(x[4 * k4 + 1].y, w.y, ay);
-                  ax = fmaf(x[4 * k4 + 2].x, w.z, ax);
-                  ay = fmaf(x[4 * k4 + 2].y, w.z, ay);
-                  ax = fmaf(x[4 * k4 + 3].x, w.w, ax);
-                  ay = fmaf(x[4 * k4 + 3].y, w.w, ay);
-                }
-              }
-              col[m * SP] = make_float2(ax, ay);
-            }
-          }
+          const float *U = s_U + (slot == 1 ? 0 : g40 * g40);
+          mx_unit<GB, SP, NL>(stg, line0, g, p, U, [&](int m, C c) { stg[(line0 + m) * SP + p] = c; });
         }
       }
     }
@@ -647,59 +672,36 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
     {
       const bool act = L < nl;
       C *row = stg + L * SP;
-      const TB *ph = A.dR + (size_t)(act ? lam : 0) * N;
       C v[N2];
+      TB phv[N2];
+      if (act) {
+        const TB *ph = A.dR + (size_t)lam * N;
+#pragma unroll
+        for (int m = 0; m < N2; m++) phv[m] = ph[a + N1 * m];
+      }
       if (act) {
 #pragma unroll
-        for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], phv[m], false);
       }
       rf_line<float, N1, N2>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
       if (act) {
 #pragma unroll
-        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], phv[m], false);
       }
     }
     __syncthreads();
-    // l -> K and store: c_K = sum_m U[K][m] c_m  (rows of U)
-    for (int u0 = 0; u0 < nu; u0 += UPR) {
-      const int u = u0 + t / N, p = t % N;
-      if (u < nu) {
+    // l -> K and store: c_K = sum_m U[K][m] c_m (rows of U^T), straight to HBM
+    for (int i0 = 0; i0 < nu * N; i0 += NT) {
+      const int i = i0 + t;
+      if (i < nu * N) {
+        const int u = i / N, p = i % N;
         const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
         const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
-        const C *col = stg + line0 * SP + p;
         if (slot) {
-          const int g4 = slot == 1 ? g40 : g41;
-          const float *U = s_U + (slot == 1 ? 0 : g40 * g40);
-          C x[GB];
-#pragma unroll
-          for (int m = 0; m < GB; m++) {
-            if (m < g) x[m] = col[m * SP];
-            else x[m] = make_float2(0.f, 0.f);
-          }
-#pragma unroll
-          for (int K = 0; K < GB; K++) {
-            if (K < g) {
-              const float *row = U + K * g4;
-              float ax = 0.f, ay = 0.f;
-#pragma unroll
-              for (int m4 = 0; m4 < GB / 4; m4++) {
-                if (4 * m4 < g) {
-                  const float4 w = *reinterpret_cast<const float4 *>(row + 4 * m4);
-                  ax = fmaf(x[4 * m4].x, w.x, ax);
-                  ay = fmaf(x[4 * m4].y, w.x, ay);
-                  ax = fmaf(x[4 * m4 + 1].x, w.y, ax);
-                  ay = fmaf(x[4 * m4 + 1].y, w.y, ay);
-                  ax = fmaf(x[4 * m4 + 2].x, w.z, ax);
-                  ay = fmaf(x[4 * m4 + 2].y, w.z, ay);
-                  ax = fmaf(x[4 * m4 + 3].x, w.w, ax);
-                  ay = fmaf(x[4 * m4 + 3].y, w.w, ay);
-                }
-              }
-              A.data[s_off[line0 + K] + p] = make_float2(ax, ay);
-            }
-          }
+          const float *UT = s_UT + (slot == 1 ? 0 : g40 * g40);
+          mx_unit<GB, SP, NL>(stg, line0, g, p, UT, [&](int K, C c) { A.data[s_off[line0 + K] + p] = c; });
         } else {
-          for (int K = 0; K < g; K++) A.data[s_off[line0 + K] + p] = col[K * SP];
+          for (int K = 0; K < g; K++) A.data[s_off[line0 + K] + p] = stg[(line0 + K) * SP + p];
         }
       }
     }
