@@ -44,11 +44,11 @@ B_ALG = {
     "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
 }
 # HBM bytes per launch and grid point, measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/hbm_traffic_*.md
+# WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/*.md.  "theta"
+# is the mean of theta-fwd and theta-bwd (the event class averages both).
 MEASURED_B_PER_POINT = {
-    "theta": (24.2 + 16.2) / 2,   # mean of theta-fwd and theta-bwd (the event class averages both)
-    "fft_R": 16.7,
-    "fft_r": 16.5,
+    "batched": {"theta": (24.0 + 16.1) / 2, "fft_R": 16.2, "fft_r": 16.2},   # batched_step_kernels_FH2_J0-20.md
+    "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
 KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER))
@@ -121,6 +121,42 @@ def cpu_baseline(prob, seconds_target=15.0):
                       % (n, prob.n1, prob.n2, prob.ntheta, cores, dt)}
 
 
+def accuracy_report(qd, out_path=None):
+    """Per-energy |P_mixed(E) - P_fp64(E)| over whole propagations (10000
+    steps of 5 au): the F+H2 J = 0 256 x 192 x 120 grid, and J = 0, 10, 20 of
+    the batched workload (all K blocks, Coriolis-coupled, j_max = 60)."""
+    import numpy as np
+
+    pb = qd.problem
+    rep = {}
+    prob = pb.config(2)
+    P = {prec: qd.run(prob, precision=prec)[1] for prec in ("fp64", "mixed")}
+    d = np.abs(P["mixed"] - P["fp64"])
+    rep["single_grid"] = {"workload": "F+H2 J=0 256x192x120, 10000 x 5 au", "max_abs_dP": float(d.max()),
+                          "at_E_eV": float(prob.collision_energies_ev[d.argmax()]), "max_P": float(P["fp64"].max()),
+                          "E_eV": prob.collision_energies_ev.tolist(), "P_fp64": P["fp64"].tolist(),
+                          "abs_dP": d.tolist()}
+    jk = pb.make_jk_problem(pb.config(2, ntheta=61), pb.F_H2, [0, 10, 20], jmax=60)
+    PJ = {}
+    for prec in ("fp64", "mixed"):
+        with qd.propagator.JKPropagator(jk, precision=prec) as prop:
+            for k in range(10):
+                prop.propagate(jk.base.total_steps // 10)
+                print("accuracy: batched %s %d0%%" % (prec, k + 1), file=sys.stderr, flush=True)
+            PJ[prec] = prop.crp_by_J()
+    rep["batched"] = {"workload": "F+H2 J=0,10,20, K=0..J, j_max=60, 256x192x(61-K), 10000 x 5 au",
+                      "per_J": {}}
+    for J in PJ["fp64"]:
+        dj = np.abs(PJ["mixed"][J] - PJ["fp64"][J])
+        rep["batched"]["per_J"][int(J)] = {"max_abs_dP": float(dj.max()), "max_P": float(PJ["fp64"][J].max()),
+                                           "abs_dP": dj.tolist(), "P_fp64": PJ["fp64"][J].tolist()}
+    if out_path:
+        with open(out_path, "w") as f:
+            json.dump(rep, f)
+    return {"metric": "per-energy |P_mixed - P_fp64|", "single_grid_max_abs_dP": rep["single_grid"]["max_abs_dP"],
+            "batched_max_abs_dP": {J: v["max_abs_dP"] for J, v in rep["batched"]["per_J"].items()}}
+
+
 def calibrate_copy(mib=1024, reps=5):
     """Copies `mib` MiB device to device `reps` times: the reference reading
     for the FETCH_SIZE / WRITE_SIZE counters (each launch moves exactly
@@ -137,7 +173,7 @@ def calibrate_copy(mib=1024, reps=5):
     print(json.dumps({"calibration_copy_MiB": mib, "reps": reps}))
 
 
-def roofline(kern, npt, prec="mixed"):
+def roofline(kern, npt, workload, prec="mixed"):
     """Dominant kernel class of the step, its algorithmic bytes per launch
     over its event-timed launch duration."""
     dom = max(("theta", "fft_R", "fft_r"), key=lambda k: kern[k])
@@ -145,8 +181,11 @@ def roofline(kern, npt, prec="mixed"):
     per_point = {"theta": (b["theta_fwd"] + b["theta_bwd"]) / 2.0, "fft_R": b["fft_R"], "fft_r": b["fft_r"]}[dom]
     achieved = per_point * npt / (kern[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[dom] * npt,
-            "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[workload][dom] * npt,
+            "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt,
+            "note": ("the R pass with K<->l mixing is VALU-bound (about 178 VALU instructions per point: the "
+                     "register FFT plus the mixing in both directions), not HBM-bound; it moves its 16 algorithmic "
+                     "bytes per point" if (dom == "fft_R" and workload == "batched") else "")}
 
 
 def main():
@@ -155,7 +194,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("batched", "single"), default="batched")
+    ap.add_argument("--workload", choices=("batched", "single", "jinterp", "accuracy"), default="batched",
+                    help="jinterp: the learned J interpolation end to end (F+H2, every 4th J of 0..40 propagated, "
+                         "the network trained on the GPU, judged against propagating all 41 J)")
+    ap.add_argument("--jinterp-steps", type=int, default=10000, help="propagation steps of the jinterp workload")
+    ap.add_argument("--out", help="jinterp: also write the full report (per-energy sums) as JSON here")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a device-to-device copy of known size (HBM counter calibration)")
     ap.add_argument("--only", choices=("batched-mixed", "single-mixed"),
@@ -164,6 +207,24 @@ def main():
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
+        return
+    if args.workload == "accuracy":
+        print(json.dumps(accuracy_report(importlib.import_module("quantum-dynamics_amd"), args.out)))
+        return
+    if args.workload == "jinterp":
+        qd = importlib.import_module("quantum-dynamics_amd")
+        ji = importlib.import_module("quantum-dynamics_amd.jinterp")
+        pb = qd.problem
+        base = pb.config(2, ntheta=61)
+        rep = ji.learned_partial_waves(qd, base, pb.F_H2, J_max=40, stride=4, jmax=60, steps=args.jinterp_steps,
+                                       reference="all", log=lambda m: print(m, file=sys.stderr, flush=True))
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(rep, f)
+        rep.pop("per_energy", None)
+        print(json.dumps({"metric": "learned J interpolation: partial-wave-sum error and wall time vs all 41 J",
+                          "workload": "F+H2 J=0..40 (every 4th propagated), K=0..min(J,60), j_max=60, "
+                                      "256x192x(61-K), mixed precision", **rep}))
         return
     if args.only:
         qd = importlib.import_module("quantum-dynamics_amd")
@@ -200,7 +261,7 @@ def main():
         "plain_fp64": {"value": npt1 / (s_plain["wall_ms_per_step"] * 1e-3), **s_plain},
     }
     single["speedup_mixed_vs_plain_fp64"] = single["mixed"]["value"] / single["plain_fp64"]["value"]
-    single["roofline"] = roofline(s_mixed["kernel_ms"], npt1)
+    single["roofline"] = roofline(s_mixed["kernel_ms"], npt1, "single")
 
     if args.workload == "batched":
         jk = batched_problem(qd)
@@ -218,7 +279,7 @@ def main():
                  "per_theta_size_ms": plain_per_n, "blocks_per_theta_size": counts}
         fused = {"value": npt / (f64["wall_ms_per_step"] * 1e-3), "ms_per_step": f64["wall_ms_per_step"]}
         kern = prof["kernel_ms"]
-        rf = roofline(kern, npt)
+        rf = roofline(kern, npt, "batched")
         cfg = {"workload": label, "blocks": len(jk.blocks), "n1": jk.base.n1, "n2": jk.base.n2,
                "theta_sizes": "41..61", "grid_points": npt, "dt": jk.base.dt}
     else:

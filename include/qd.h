@@ -121,6 +121,11 @@ typedef struct qd_options {
                              products) or 3 (six products); 0 = 3 */
   int32_t autotune;       /* 1: time the candidate kernel configurations on the
                              handle's own arrays at creation and keep the fastest */
+  int32_t fuse_theta;     /* mixed precision, CPU-propagator order, angular basis of at
+                             most 64: 0 = fuse the theta-backward GEMM of each step with
+                             the dump, the flux rows, the V phase and the next
+                             theta-forward GEMM (the state between steps is then the FBR
+                             array; psi is rebuilt from it when read); -1 = never */
 } qd_options;
 
 /* Per-step callback, the equivalent of options.wave_to_matlab
@@ -200,7 +205,8 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * group, [3] register-FFT R pass, [4] register-FFT r pass, [5] R-pass lines
  * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
- * [9] its unit-size bucket.  vals holds 10 entries. */
+ * [9] its unit-size bucket, [10] fused theta step in use, [11] its 16-row
+ * tiles per work group.  vals holds 12 entries. */
 int qd_get_tuning(qd_handle *h, int32_t *vals);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

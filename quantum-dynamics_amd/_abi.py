@@ -48,7 +48,7 @@ class qd_options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("pipeline", C.c_int32), ("variant", C.c_int32),
                 ("flux_block", C.c_int32), ("crp_every", C.c_int32), ("use_graph", C.c_int32),
                 ("device", C.c_int32), ("profile", C.c_int32),
-                ("theta_split", C.c_int32), ("autotune", C.c_int32)]
+                ("theta_split", C.c_int32), ("autotune", C.c_int32), ("fuse_theta", C.c_int32)]
 
 
 class qd_jk_block(C.Structure):

@@ -212,6 +212,12 @@ struct qd_handle {
   float *mx_umat = nullptr;
   long long *line_off = nullptr;
   size_t tmx_lds = 0;
+  // fused theta step (k_theta_fused): between steps the state is the FBR array
+  // holding theta-forward(V psi); psi itself is rebuilt from it when read
+  bool fuse_theta = false, state_fbr = false, dvr_valid = true;
+  int fuse_opt = 0, tfu_tpw = 1;
+  size_t tfu_lds = 0;
+
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
   FftPlanArg planR{}, planr{};
@@ -445,6 +451,51 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   return QD_OK;
 }
 
+// rows = false leaves psi untouched (no flux rows), for the autotuner
+int launch_theta_fused(qd_handle *h, bool rows = true) {
+  ThetaFusedArgs F;
+  F.L = h->lpsi32;
+  F.psi = h->psi32;
+  F.Bb = h->Wb;
+  F.Bf = h->Wf;
+  F.mat_stride = h->mat_stride;
+  F.blocks = h->d_blocks;
+  F.R = (int)h->n12;
+  F.expV = h->expV32;
+  F.dcol_vidx = h->dcol_vidx;
+  F.d1f = h->has_dump ? h->d1f : nullptr;
+  F.d2f = h->has_dump ? h->d2f : nullptr;
+  F.n1 = h->n1;
+  F.r_lo = h->do_crp && rows ? h->surf - h->stencil.h : 1;
+  F.r_hi = h->do_crp && rows ? h->surf + h->stencil.h : 0;
+  F.tpw = h->tfu_tpw;
+  size_t lds = 0;
+  const void *fn = theta_fused_kernel(h->tmx_ks, &lds);
+  if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
+  const long long ntiles = (h->n12 + 15) / 16;
+  dim3 grid((unsigned)((ntiles + h->tfu_tpw - 1) / h->tfu_tpw), 1, h->nb);
+  void *args[] = {&F};
+  QD_HIP(hipLaunchKernel(fn, grid, dim3(TMX_THREADS), args, lds, h->stream));
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+// Into the fused state: the FBR array becomes theta-forward(V psi).
+int enter_fbr_state(qd_handle *h) {
+  if (!h->fuse_theta || h->state_fbr) return QD_OK;
+  QD_RC(launch_theta(h, true, 1, 0));
+  h->state_fbr = true;
+  return QD_OK;
+}
+// psi from the fused state: conj(exp(-i V dt)) times theta-backward of the
+// FBR array (the forward and backward transforms are exact inverses).
+int materialize_psi(qd_handle *h) {
+  if (!h->fuse_theta || !h->state_fbr || h->dvr_valid) return QD_OK;
+  QD_RC(launch_theta(h, false, 0, 2));
+  h->dvr_valid = true;
+  return QD_OK;
+}
+
 int flush_flux(qd_handle *h) {
   if (!h->do_crp || h->slot == 0) return QD_OK;
   QD_RC(mark(h, TK_FLUX));
@@ -502,7 +553,17 @@ int step_impl(qd_handle *h) {
   C *lpsi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32);
   const bool cuda = h->variant == QD_VARIANT_CUDA;
   const bool dump = h->has_dump;
-  if (h->pipe == QD_PIPE_FUSED) {
+  if (h->fuse_theta) {
+    // state on entry: the FBR array holds theta-forward(V psi_n); on exit the
+    // same for psi_{n+1}.  The psi rows of the flux surface are written by the
+    // fused kernel, after the dump, for k_flux_extract below.
+    QD_RC(mark(h, TK_FFTR1));
+    QD_RC(launch_rpass<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
+    QD_RC(mark(h, TK_FFTR2));
+    QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
+    QD_RC(mark(h, TK_THETA));
+    QD_RC(launch_theta_fused(h));
+  } else if (h->pipe == QD_PIPE_FUSED) {
     // CUDA propagator: dump at the start of the step (evolutionCUDA2.cu:503),
     // fused with the potential phase; CPU propagator: dump after the step.
     QD_RC(mark(h, TK_THETA));
@@ -592,6 +653,8 @@ int set_options(qd_handle *h, const qd_options *o) {
   h->crp_every = opt.crp_every;
   h->profile = opt.profile;
   h->nsplit = opt.theta_split == 2 ? 2 : 3;
+  if (opt.fuse_theta != 0 && opt.fuse_theta != -1) return fail(QD_ERR_ARG, "fuse_theta must be 0 or -1");
+  h->fuse_opt = opt.fuse_theta;
   return QD_OK;
 }
 
@@ -660,7 +723,7 @@ int setup_theta_mx(qd_handle *h) {
   QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
   int per_cu = 1;
   for (int pro = 0; pro <= 2; pro++)
-    for (int epi = 0; epi <= 1; epi++) {
+    for (int epi = 0; epi <= 2; epi++) {
       const void *fn = theta_mx_kernel(h->tmx_ks, pro, epi);
       if (!fn) continue;
       QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->tmx_lds));
@@ -1250,6 +1313,20 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMemcpy(db, B.data(), B.size() * 2, hipMemcpyHostToDevice));
     h->Wf = df;
     h->Wb = db;
+    // fused theta step: CPU-propagator order, three pieces, both matrices in LDS
+    size_t flds = 0;
+    if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->variant == QD_VARIANT_CPU && h->nsplit == 3 &&
+        h->tmx_ks >= 1 && h->tmx_ks <= 2 && theta_fused_kernel(h->tmx_ks, &flds)) {
+      const void *fn = theta_fused_kernel(h->tmx_ks, &flds);
+      QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+      int nbk = 0;
+      QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, TMX_THREADS, flds));
+      if (nbk >= 1) {
+        h->fuse_theta = true;
+        h->tfu_lds = flds;
+        h->tfu_tpw = h->tmx_tpw;
+      }
+    }
   }
 
   if (h->do_crp) {
@@ -1474,9 +1551,12 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
     h->t_n[k] = 0;
   }
   if (h->pending_prephase) {
+    QD_RC(materialize_psi(h));
     QD_RC(apply_prephase(h));
     h->pending_prephase = false;
   }
+  QD_RC(enter_fbr_state(h));
+  if (h->fuse_theta) h->dvr_valid = false;
   hipEvent_t e0 = h->ev[0], e1 = h->ev[1];
   h->nmarks = 0;
   QD_HIP(hipEventRecord(e0, h->stream));
@@ -1531,6 +1611,7 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
 
 int get_psi_impl(qd_handle *h, double *out) {
   QD_HIP(hipSetDevice(h->dev));
+  QD_RC(materialize_psi(h));
   if (h->prec == QD_PREC_FP64) {
     QD_HIP(hipMemcpyAsync(out, h->psi64, sizeof(double2) * h->npt, hipMemcpyDeviceToHost, h->stream));
   } else {
@@ -1547,6 +1628,7 @@ int get_psi_impl(qd_handle *h, double *out) {
 
 int norm_impl(qd_handle *h, double *out) {
   QD_HIP(hipSetDevice(h->dev));
+  QD_RC(materialize_psi(h));
   if (h->prec == QD_PREC_FP64)
     hipLaunchKernelGGL(k_norm_partial<double>, dim3(h->npartial), dim3(256), 0, h->stream, h->psi64, h->dcol_w,
                        h->n12, h->npt, h->partial);
@@ -1603,6 +1685,25 @@ int autotune_handle(qd_handle *h) {
       }
     }
     h->tmx_tpw = pick;
+  }
+  // fused theta kernel: the same choice of tiles per work group
+  if (h->fuse_theta) {
+    const long long ntiles = (h->n12 + 15) / 16;
+    const int base = h->tfu_tpw;
+    std::vector<int> cands = {base, std::max(1, base / 2), base * 3 / 2, base * 2, base * 4};
+    float best = 1e30f;
+    int pick = base;
+    for (int c : cands) {
+      if (c < 1 || c > ntiles) continue;
+      h->tfu_tpw = c;
+      float ms = 0;
+      QD_RC(time_launches(h, [&] { return launch_theta_fused(h, false); }, reps, &ms));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    h->tfu_tpw = pick;
   }
   // R and r passes: register FFT or Stockham kernels
   auto pick_pass = [&](bool *flag, bool alongR) -> int {
@@ -1697,6 +1798,8 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[7] = h->rl_lpb;
   vals[8] = h->rp_mx ? h->mx_nl : 0;
   vals[9] = h->rp_mx ? h->mx_gb : 0;
+  vals[10] = h->fuse_theta ? 1 : 0;
+  vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
   return QD_OK;
 }
 
@@ -1808,6 +1911,8 @@ int qd_set_psi(qd_handle *h, const double *psi_in) {
     QD_HIP(hipStreamSynchronize(h->stream));
     QD_HIP(hipFree(tmp));
   }
+  h->state_fbr = false;
+  h->dvr_valid = true;
   return QD_OK;
 }
 

@@ -177,6 +177,7 @@ const void *theta_mx_kernel(int ks, int pro, int epi) {
     if (pro == 1 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 1, 0>); \
     if (pro == 2 && epi == 0) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 2, 0>); \
     if (pro == 0 && epi == 1) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 1>); \
+    if (pro == 0 && epi == 2) return reinterpret_cast<const void *>(&k_theta_mx<KS_, 2 * KS_, 0, 2>); \
   }
   QD_TMX(1)
   QD_TMX(2)
@@ -186,6 +187,14 @@ const void *theta_mx_kernel(int ks, int pro, int epi) {
   return nullptr;
 }
 
+// fused FBR -> DVR -> FBR kernel for KS k-steps (both matrices in LDS)
+const void *theta_fused_kernel(int ks, size_t *lds) {
+  switch (ks) {
+    case 1: *lds = theta_fused_lds<1>(); return reinterpret_cast<const void *>(&k_theta_fused<1>);
+    case 2: *lds = theta_fused_lds<2>(); return reinterpret_cast<const void *>(&k_theta_fused<2>);
+    default: return nullptr;
+  }
+}
 #endif  // theta part
 
 #if QD_PART == 5 || QD_PART == 6

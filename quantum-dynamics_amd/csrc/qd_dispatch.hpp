@@ -31,6 +31,9 @@ const void *rline_rf_kernel(int n, size_t *lds, int *lpb);
 const void *rpass_mx_kernel(int n, int nl, int gb, size_t *lds_fixed);
 const void *rpass_mx_kernel_small(int n, int nl, int gb, size_t *lds_fixed);
 // matrix-stationary mixed theta GEMM for ks k-steps, V-phase prologue pro, dump epilogue epi
+// (epi 2: FBR -> DVR times conj(exp(-i V dt)), the DVR state from the fused step's FBR state)
 const void *theta_mx_kernel(int ks, int pro, int epi);
+// fused FBR -> DVR -> FBR theta kernel (k_theta_fused) for ks k-steps; *lds = dynamic LDS bytes
+const void *theta_fused_kernel(int ks, size_t *lds);
 
 }  // namespace qd

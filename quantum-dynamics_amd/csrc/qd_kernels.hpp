@@ -1004,6 +1004,10 @@ __global__ __launch_bounds__(512) void k_theta_mx(ThetaArgs P) {
   const __amdgpu_buffer_rsrc_t rV =
       buf_rsrc(PRO ? reinterpret_cast<const float2 *>(P.expV) + P.dcol_vidx[acol0] : nullptr, PRO ? (unsigned)K * R * 8u : 0u);
   const __amdgpu_buffer_rsrc_t rC = buf_rsrc(reinterpret_cast<float2 *>(P.C) + ccol0 * R, (unsigned)N * R * 8u);
+  // EPI == 2 (FBR -> DVR): the output columns times conj(exp(-i V dt))
+  const __amdgpu_buffer_rsrc_t rVo =
+      buf_rsrc(EPI == 2 ? reinterpret_cast<const float2 *>(P.expV) + P.dcol_vidx[ccol0] : nullptr,
+               EPI == 2 ? (unsigned)N * R * 8u : 0u);
   {
     const __bf16 *Mg = reinterpret_cast<const __bf16 *>(P.B) + (size_t)bd.mat * P.mat_stride;
     constexpr int QV = KP / 8;  // bf8v per matrix row
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(512) void k_theta_mx(ThetaArgs P) {
   for (; tile < t1; tile += W) {
     const int row = tile * 16 + rl;
     float dfac = 1.f;
-    if ((PRO == 2 || EPI) && row < R) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
+    if ((PRO == 2 || EPI == 1) && row < R) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
     f4v acc[2][NT];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1096,13 +1100,220 @@ __global__ __launch_bounds__(512) void k_theta_mx(ThetaArgs P) {
       for (int v = 0; v < 4; v++) {
         const int n = nt * 16 + 4 * kg + v;
         float2 c = make_float2(acc[0][nt][v], acc[1][nt][v]);
-        if (EPI) {
+        const unsigned off = (row < R && n < N) ? (unsigned)(row + R * n) * 8u : TM_OOB;
+        if (EPI == 1) {
           c.x *= dfac;
           c.y *= dfac;
         }
-        buf_st_f2(c, rC, (row < R && n < N) ? (unsigned)(row + R * n) * 8u : TM_OOB);
+        if (EPI == 2) {
+          const float2 w = buf_ld_f2(rVo, off);
+          c = make_float2(c.x * w.x + c.y * w.y, c.y * w.x - c.x * w.y);
+        }
+        buf_st_f2(c, rC, off);
       }
   }
+}
+
+// Fused FBR -> DVR -> FBR theta kernel (mixed precision, three BF16 pieces,
+// matrix-stationary like k_theta_mx) for the CPU-propagator step order: the
+// theta-backward GEMM of step n, the dump, the DVR values of the flux-surface
+// rows r_lo..r_hi (written to psi for k_flux_extract; no other DVR value is
+// stored), the V phase and the theta-forward GEMM of step n + 1, in place on
+// the FBR array.  Each wave owns 16-row tiles; lane (rl, kg) holds row rl.
+// The backward GEMM leaves DVR column nt * 16 + 4 kg + v in lane (rl, kg); the
+// forward GEMM wants columns 32 s + 8 kg + e.  The values pass through a
+// per-wave LDS tile to get there.  (Feeding them straight from the registers
+// with the forward matrix's k index permuted instead groups different columns
+// into each MFMA k-block, and that grouping was measured to gain norm
+// coherently, 6e-8 per step; the natural grouping of the unfused kernels
+// does not.)
+struct ThetaFusedArgs {
+  float2 *L;                   // FBR array, in and out
+  float2 *psi;                 // DVR array: the flux rows are written
+  const void *Bb, *Bf;         // packed backward and forward matrices, mat_stride apart
+  long long mat_stride;
+  const BlockDesc *blocks;
+  int R;                       // n1 * n2
+  const float2 *expV;          // exp(-i V dt) per V column
+  const long long *dcol_vidx;
+  const float *d1f, *d2f;      // dump factors (nullptr: no dump)
+  int n1;
+  int r_lo, r_hi;              // flux rows (r_lo > r_hi: none)
+  int tpw;
+};
+
+template <int KS>
+__global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
+  extern __shared__ __align__(16) unsigned char tm_lds[];
+  constexpr int KP = KS * 32, NT = 2 * KS, NP = NT * 16, LD = KP + 8, XS = KP + 1;
+  __bf16 *Mb = reinterpret_cast<__bf16 *>(tm_lds);  // [3][NP][LD], row n = DVR column, k = FBR column
+  __bf16 *Mf = Mb + 3 * NP * LD;                     // [3][NP][LD], row n = FBR column, k = DVR column
+  // per wave, the tile's DVR values [16][XS] in natural column order
+  float2 *Xw = reinterpret_cast<float2 *>(Mf + 3 * NP * LD) + (threadIdx.x >> 6) * 16 * XS;
+  const BlockDesc bd = P.blocks[blockIdx.z];
+  const int R = P.R, nd = bd.nd, nf = bd.nf;
+  const __amdgpu_buffer_rsrc_t rL = buf_rsrc(P.L + (size_t)bd.fcol0 * R, (unsigned)nf * R * 8u);
+  const __amdgpu_buffer_rsrc_t rP = buf_rsrc(P.psi + (size_t)bd.dcol0 * R, (unsigned)nd * R * 8u);
+  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(P.expV + P.dcol_vidx[bd.dcol0], (unsigned)nd * R * 8u);
+  {
+    const __bf16 *Gb = reinterpret_cast<const __bf16 *>(P.Bb) + (size_t)bd.mat * P.mat_stride;
+    const __bf16 *Gf = reinterpret_cast<const __bf16 *>(P.Bf) + (size_t)bd.mat * P.mat_stride;
+    constexpr int QV = KP / 8;
+    for (int u = threadIdx.x; u < 3 * NP * QV; u += blockDim.x) {
+      const int pn = u / QV, kq = u % QV;
+      *reinterpret_cast<bf8v *>(&Mb[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&Gb[(size_t)pn * KP + kq * 8]);
+      *reinterpret_cast<bf8v *>(&Mf[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&Gf[(size_t)pn * KP + kq * 8]);
+    }
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int rl = lane & 15, kg = lane >> 4;
+  const int ntiles = (R + 15) >> 4;
+  const int t1 = min(ntiles, (int)(blockIdx.x + 1) * P.tpw);
+  float2 ra[8];
+  auto fetch = [&](int tile, int s) {
+    const int row = tile * 16 + rl;
+    const int k0 = s * 32 + kg * 8;
+    const unsigned base = (row < R ? (unsigned)row * 8u : TM_OOB) + (unsigned)(R * k0) * 8u;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ra[e] = buf_ld_f2(rL, k0 + e < nf ? base + (unsigned)(R * e) * 8u : TM_OOB);
+  };
+  // three BF16 pieces of 8 FP32 values
+  auto split3 = [](const float *x, bf8v *out) {
+    float r[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = x[e];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const __bf16 b = f2bf(r[e]);
+        out[q][e] = b;
+        r[e] -= (float)b;
+      }
+  };
+  int tile = blockIdx.x * P.tpw + wave;
+  if (tile < t1) fetch(tile, 0);
+  for (; tile < t1; tile += W) {
+    const int row = tile * 16 + rl;
+    const bool rin = row < R;
+    // V phase of this tile's DVR columns, loaded ahead of the backward GEMM
+    float2 wv[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int n = nt * 16 + 4 * kg + v;
+        wv[nt][v] = buf_ld_f2(rV, (rin && n < nd) ? (unsigned)(row + R * n) * 8u : TM_OOB);
+      }
+    f4v acc[2][NT];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < NT; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    // backward GEMM: DVR[n] = sum_k Pb[n][k] FBR[k]
+#pragma unroll 1
+    for (int s = 0; s < KS; s++) {
+      bf8v bre[3], bim[3];
+      {
+        float re[8], im[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          re[e] = ra[e].x;
+          im[e] = ra[e].y;
+        }
+        split3(re, bre);
+        split3(im, bim);
+      }
+      {
+        const bool last = s + 1 == KS;
+        const int ft = last ? tile + W : tile;
+        if (ft < t1) fetch(ft, last ? 0 : s + 1);
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        bf8v a[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++) a[p] = *reinterpret_cast<const bf8v *>(&Mb[(p * NP + nt * 16 + rl) * LD + s * 32 + kg * 8]);
+#pragma unroll
+        for (int d = 2; d >= 0; d--)
+#pragma unroll
+          for (int p = 0; p <= d; p++) {
+            acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bre[d - p], acc[0][nt], 0, 0, 0);
+            acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bim[d - p], acc[1][nt], 0, 0, 0);
+          }
+      }
+    }
+    // DVR domain: dump, flux rows, V phase; then the pieces of the forward B operand
+    float dfac = 1.f;
+    if (P.d1f && rin) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
+    const int rr = row / P.n1;
+    const bool frow = rin && rr >= P.r_lo && rr <= P.r_hi;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int n = nt * 16 + 4 * kg + v;
+        const float2 c = make_float2(acc[0][nt][v] * dfac, acc[1][nt][v] * dfac);
+        if (frow) buf_st_f2(c, rP, n < nd ? (unsigned)(row + R * n) * 8u : TM_OOB);
+        const float2 w = wv[nt][v];
+        Xw[rl * XS + n] = make_float2(c.x * w.x - c.y * w.y, c.x * w.y + c.y * w.x);
+      }
+    // the wave's own LDS rows: its writes land before its reads, which take
+    // the columns in the forward GEMM's order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    bf8v fre[KS][3], fim[KS][3];
+#pragma unroll
+    for (int s2 = 0; s2 < KS; s2++) {
+      float re[8], im[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float2 x = Xw[rl * XS + s2 * 32 + kg * 8 + e];
+        re[e] = x.x;
+        im[e] = x.y;
+      }
+      split3(re, fre[s2]);
+      split3(im, fim[s2]);
+    }
+    // reads done before the next tile's writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // forward GEMM: FBR[m] = sum_n Wf[n][m] DVR[n]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < NT; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s2 = 0; s2 < KS; s2++)
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        bf8v a[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++) a[p] = *reinterpret_cast<const bf8v *>(&Mf[(p * NP + nt * 16 + rl) * LD + s2 * 32 + kg * 8]);
+#pragma unroll
+        for (int d = 2; d >= 0; d--)
+#pragma unroll
+          for (int p = 0; p <= d; p++) {
+            acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[s2][d - p], acc[0][nt], 0, 0, 0);
+            acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[s2][d - p], acc[1][nt], 0, 0, 0);
+          }
+      }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = nt * 16 + 4 * kg + v;
+        buf_st_f2(make_float2(acc[0][nt][v], acc[1][nt][v]), rL, (rin && m < nf) ? (unsigned)(row + R * m) * 8u : TM_OOB);
+      }
+  }
+}
+template <int KS>
+constexpr size_t theta_fused_lds() {
+  return (size_t)2 * 3 * (2 * KS * 16) * (KS * 32 + 8) * 2 + (size_t)(TMX_THREADS / 64) * 16 * (KS * 32 + 1) * 8;
 }
 
 template <int KS, int NT>

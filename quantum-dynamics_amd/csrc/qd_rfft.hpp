@@ -515,9 +515,10 @@ __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, in
   float2 acc[G];
 #pragma unroll
   for (int n = 0; n < G; n++) acc[n] = make_float2(0.f, 0.f);
-  // two k-steps per iteration: enough LDS reads in flight to cover their
-  // latency without holding the whole matrix in registers
-#pragma unroll 2
+  // two k-steps per iteration (one above 32 lines): enough LDS reads in
+  // flight to cover their latency without holding the whole matrix in registers
+  constexpr int UK = G > 32 ? 1 : 2;
+#pragma unroll UK
   for (int k = 0; k < G; k++) {
     const float2 xr = stg[min(line0 + k, NL - 1) * SP + p];
     const float2 x = make_float2(k < g ? xr.x : 0.f, k < g ? xr.y : 0.f);
@@ -621,12 +622,17 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
     }
   };
 
+  // With 48-line bundles (768 threads, at most 168 registers per thread) the
+  // lines are loaded at the top of their own iteration instead of one bundle
+  // ahead, which keeps the 48 mixing accumulators in registers.
+  constexpr bool PREF = NT <= 512;
   int b = blockIdx.x;
   if (b < A.nbundle) {
     fetch_desc(b, true);
-    fetch_data();
+    if (PREF) fetch_data();
   }
   for (; b < A.nbundle; b += gridDim.x) {
+    if (!PREF) fetch_data();
     const int nl = __builtin_amdgcn_readfirstlane(h0.x), nu = __builtin_amdgcn_readfirstlane(h0.y);
     const int g40 = __builtin_amdgcn_readfirstlane(h1.x), g41 = __builtin_amdgcn_readfirstlane(h1.y);
     const int lam = d_lam;
@@ -667,7 +673,7 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
     }
     __syncthreads();
     // the next bundle's lines and matrices stream in during the rest
-    fetch_data();
+    if (PREF) fetch_data();
     // FFT phase: line L in the l basis
     {
       const bool act = L < nl;
@@ -685,8 +691,9 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
       }
       rf_line<float, N1, N2>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
       if (act) {
+        const TB *ph = A.dR + (size_t)lam * N;
 #pragma unroll
-        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], phv[m], false);
+        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], PREF ? phv[m] : ph[a + N1 * m], false);
       }
     }
     __syncthreads();

@@ -203,3 +203,103 @@ class JInterpolator:
 
     def close(self):
         self.net.close()
+
+
+def propagate_pj(qd, base, system, Js, jmax: int, steps: int, chunk: int = 1000, log=None, **jk_kw):
+    """P^J(E) of every J in Js from one batched mixed-precision propagation
+    (all K blocks of every J, Coriolis-coupled; each J normalised by the
+    packet's |eta|^2 measured from its centrifugal energy), and the seconds it
+    took:
+    problem set-up, handle creation (upload, tables, autotuning) and the
+    propagation itself."""
+    from . import problem as pbm
+    from .propagator import JKPropagator
+
+    t0 = time.perf_counter()
+    jk = pbm.make_jk_problem(base, system, list(Js), jmax=jmax, **jk_kw)
+    t1 = time.perf_counter()
+    with JKPropagator(jk, precision="mixed", autotune=1) as prop:
+        t2 = time.perf_counter()
+        done = 0
+        while done < steps:
+            n = min(chunk, steps - done)
+            prop.propagate(n)
+            done += n
+            if log:
+                log("  J %s: %d / %d steps, %.1f s" % (_jrange(Js), done, steps, time.perf_counter() - t2))
+        PJ = prop.crp_by_J(eta="centrifugal")
+    t3 = time.perf_counter()
+    points = int(jk.psi.size)
+    del jk
+    return PJ, {"setup_s": t1 - t0, "create_s": t2 - t1, "propagate_s": t3 - t2, "total_s": t3 - t0,
+                "grid_points": points, "blocks": sum(min(J, jmax) + 1 for J in Js)}
+
+
+def _jrange(Js):
+    Js = sorted(Js)
+    return "%d..%d (%d values)" % (Js[0], Js[-1], len(Js))
+
+
+def learned_partial_waves(qd, base, system, J_max: int = 40, stride: int = 4, n_held: int = 5, jmax: int = 60,
+                          steps: Optional[int] = None, R_b: float = 3.5, iters: int = 20000, reference: str = "all",
+                          log=None, seed: int = 0) -> dict:
+    """The learned J interpolation end to end.
+
+    Propagates every `stride`-th J of 0..J_max, trains the network on their
+    P^J(E) on the GPU, predicts the skipped J, and judges the result with the
+    propagator itself: `reference` = "all" propagates every J (the partial-wave
+    sum without the network and its wall time), "held" only `n_held` skipped J
+    spread over the range.  Reports the per-energy error of the partial-wave
+    sum sum_J (2J+1) P^J(E) (the cross section up to the common factor pi/k^2)
+    for the network and for linear interpolation in J, the error on the
+    held-out J, and the wall times."""
+    steps = int(steps if steps is not None else base.total_steps)
+    J_all = list(range(J_max + 1))
+    J_train = J_all[::stride]
+    if J_train[-1] != J_max:
+        J_train.append(J_max)
+    skipped = [J for J in J_all if J not in J_train]
+    held = skipped[::max(1, len(skipped) // n_held)][:n_held]
+    E = np.asarray(base.energies, dtype=np.float64)
+
+    PJ_train, t_train = propagate_pj(qd, base, system, J_train, jmax, steps, log=log)
+    t0 = time.perf_counter()
+    model = JInterpolator(E, mu=system.mu_R, R_b=R_b, J_max=J_max, seed=seed)
+    loss = model.fit(PJ_train, iters=iters, lr=3e-3, lr_final=1e-4)
+    pred = {J: (PJ_train[J] if J in PJ_train else model.predict(J)) for J in J_all}
+    fit_s = time.perf_counter() - t0
+    model.close()
+    lin = {J: (PJ_train[J] if J in PJ_train else linear_in_J(J_train, PJ_train, J)) for J in J_all}
+
+    if reference == "all":
+        PJ_ref, t_ref = propagate_pj(qd, base, system, J_all, jmax, steps, log=log)
+    else:
+        PJ_ref, t_ref = propagate_pj(qd, base, system, held, jmax, steps, log=log)
+        PJ_ref.update(PJ_train)
+    out = {
+        "J_max": J_max, "stride": stride, "J_propagated": J_train, "J_held_out": held, "steps": steps,
+        "energies_hartree": [float(E[0]), float(E[-1]), int(E.size)],
+        "train": t_train, "fit_seconds": fit_s, "training_loss": loss, "reference": reference,
+        "reference_run": t_ref,
+    }
+    d_net = {J: np.abs(pred[J] - PJ_ref[J]) for J in held}
+    d_lin = {J: np.abs(lin[J] - PJ_ref[J]) for J in held}
+    out["held_out_max_abs_dP"] = {"network": {J: float(d_net[J].max()) for J in held},
+                                  "linear_in_J": {J: float(d_lin[J].max()) for J in held}}
+    if reference == "all":
+        ref_sum = partial_wave_sum(PJ_ref)
+        scale = float(np.max(np.abs(ref_sum)))
+        e_net = np.abs(partial_wave_sum(pred) - ref_sum)
+        e_lin = np.abs(partial_wave_sum(lin) - ref_sum)
+        out["partial_wave_sum_max"] = scale
+        out["cross_section_rel_error"] = {
+            "network": {"max": float(e_net.max() / scale), "rms": float(np.sqrt(np.mean(e_net ** 2)) / scale)},
+            "linear_in_J": {"max": float(e_lin.max() / scale), "rms": float(np.sqrt(np.mean(e_lin ** 2)) / scale)},
+        }
+        out["per_energy"] = {"E_hartree": E.tolist(), "reference": ref_sum.tolist(),
+                             "network": partial_wave_sum(pred).tolist(), "linear_in_J": partial_wave_sum(lin).tolist(),
+                             "PJ_reference": {int(J): PJ_ref[J].tolist() for J in J_all},
+                             "PJ_network": {int(J): pred[J].tolist() for J in J_all}}
+        learned = t_train["total_s"] + fit_s
+        out["wall_seconds"] = {"learned": learned, "all_J": t_ref["total_s"], "speedup": t_ref["total_s"] / learned}
+    return out

@@ -456,6 +456,21 @@ class JKProblem:
     pot: dict                      # K -> V (n1*n2*n_K flat)
     psi: np.ndarray                # complex, all blocks
     coriolis: bool = True
+    packet: Optional[dict] = None  # R0, delta, k0, e_v, mu of the initial packet
+
+    def eta_sq_J(self, J: int) -> np.ndarray:
+        """|eta(E)|^2 of the packet measured from the centrifugal energy
+        J(J+1) / (2 mu R0^2) at its centre: a plane-wave component k at R0 has
+        total energy e_v + k^2/2mu + J(J+1)/(2 mu R0^2).  The plain |eta(E)|^2
+        (base.eta_sq) ignores the shift, which at the top of the energy window
+        can push P^J(E) of large J above 1.  Zero where E is below the shift."""
+        pk = self.packet
+        shift = J * (J + 1.0) / (2.0 * pk["mu"] * pk["R0"] ** 2)
+        Ek = np.asarray(self.base.energies, dtype=np.float64) - pk["e_v"] - shift
+        out = np.zeros(Ek.size)
+        ok = Ek > 0
+        out[ok] = eta_sq(pk["delta"], pk["k0"], pk["mu"], Ek[ok])
+        return out
 
     @property
     def grid_points(self):
@@ -544,4 +559,5 @@ def make_jk_problem(base: Problem, system: LepsSystem, Js, jmax: int, j0: int = 
     jbase = base.copy()
     jbase.eta_sq = eta_sq(delta, k0, muR, base.energies - e_v)
     return JKProblem(base=jbase, jmax=jmax, Js=list(Js), blocks=blocks, x=x, omega=om, Pb=Pbm, Wf=Wfm,
-                     pot=pot, psi=psi, coriolis=coriolis)
+                     pot=pot, psi=psi, coriolis=coriolis,
+                     packet=dict(R0=R0, delta=delta, k0=k0, e_v=e_v, mu=muR))

