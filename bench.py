@@ -39,15 +39,18 @@ HBM_PEAK_GBS = 8000.0
 # (B = s_c (2 + 6 M / n_theta) + s_v per grid-point-step: 68 B mixed, 136 B
 # FP64): theta-fwd reads psi (s_c) and V (s_v) and writes the FBR array; the R
 # and r passes and theta-bwd each read and write one complex array.
+# With the fused theta step (mixed precision, basis of at most 64) one kernel
+# per step reads the FBR array and exp(-i V dt) (8 B each) and writes the FBR
+# array: 24 B per point.
 B_ALG = {
-    "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "fft_R": 16, "fft_r": 16, "total": 68},
+    "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "theta_fused": 24, "fft_R": 16, "fft_r": 16, "total": 68},
     "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
 }
 # HBM bytes per launch and grid point, measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/*.md.  "theta"
 # is the mean of theta-fwd and theta-bwd (the event class averages both).
 MEASURED_B_PER_POINT = {
-    "batched": {"theta": (24.0 + 16.1) / 2, "fft_R": 16.2, "fft_r": 16.2},   # batched_step_kernels_FH2_J0-20.md
+    "batched": {"theta": 24.15, "fft_R": 16.2, "fft_r": 16.2},             # batched_step_kernels_FH2_J0-20.md (fused theta)
     "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
@@ -173,12 +176,14 @@ def calibrate_copy(mib=1024, reps=5):
     print(json.dumps({"calibration_copy_MiB": mib, "reps": reps}))
 
 
-def roofline(kern, npt, workload, prec="mixed"):
+def roofline(kern, npt, workload, fused_theta, prec="mixed"):
     """Dominant kernel class of the step, its algorithmic bytes per launch
-    over its event-timed launch duration."""
+    over its event-timed launch duration (the theta class is one fused launch
+    per step, or the mean of the forward and backward launches)."""
     dom = max(("theta", "fft_R", "fft_r"), key=lambda k: kern[k])
     b = B_ALG[prec]
-    per_point = {"theta": (b["theta_fwd"] + b["theta_bwd"]) / 2.0, "fft_R": b["fft_R"], "fft_r": b["fft_r"]}[dom]
+    theta = b["theta_fused"] if fused_theta else (b["theta_fwd"] + b["theta_bwd"]) / 2.0
+    per_point = {"theta": theta, "fft_R": b["fft_R"], "fft_r": b["fft_r"]}[dom]
     achieved = per_point * npt / (kern[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[workload][dom] * npt,
@@ -261,7 +266,7 @@ def main():
         "plain_fp64": {"value": npt1 / (s_plain["wall_ms_per_step"] * 1e-3), **s_plain},
     }
     single["speedup_mixed_vs_plain_fp64"] = single["mixed"]["value"] / single["plain_fp64"]["value"]
-    single["roofline"] = roofline(s_mixed["kernel_ms"], npt1, "single")
+    single["roofline"] = roofline(s_mixed["kernel_ms"], npt1, "single", s_mixed["tuning"]["fused_theta"])
 
     if args.workload == "batched":
         jk = batched_problem(qd)
@@ -279,7 +284,7 @@ def main():
                  "per_theta_size_ms": plain_per_n, "blocks_per_theta_size": counts}
         fused = {"value": npt / (f64["wall_ms_per_step"] * 1e-3), "ms_per_step": f64["wall_ms_per_step"]}
         kern = prof["kernel_ms"]
-        rf = roofline(kern, npt, "batched")
+        rf = roofline(kern, npt, "batched", prof["tuning"]["fused_theta"])
         cfg = {"workload": label, "blocks": len(jk.blocks), "n1": jk.base.n1, "n2": jk.base.n2,
                "theta_sizes": "41..61", "grid_points": npt, "dt": jk.base.dt}
     else:

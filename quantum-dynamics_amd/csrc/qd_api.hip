@@ -791,41 +791,23 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
     maxu = std::max(maxu, cur.mg4[0] * cur.mg4[0] + cur.mg4[1] * cur.mg4[1]);
     cur = MxHead{};
   };
-  auto open_bundle = [&]() {
-    loff.resize(loff.size() + NL, 0);
-    llam.resize(llam.size() + NL, 0);
-    uni.resize(uni.size() + NL, 0);
-    cur.msrc[0] = cur.msrc[1] = 0;
-  };
-  for (const Group &g : S.groups) {
+  // units (group, r) of group gi, as many as fit, into the open bundle
+  std::vector<int> next_r(S.groups.size(), 0);
+  auto add_units = [&](int gi, int count) {
+    const Group &g = S.groups[gi];
     const int gs = (int)g.cols.size();
     const bool mixed = g.u0 >= 0 && gs > 1;
-    const int src = mixed ? mat_of[g.u0].first : -1, g4 = mixed ? mat_of[g.u0].second : 0;
-    for (int r = 0; r < n2; r++) {
-      int slot = 0;
-      if (cur.nl > 0) {
-        bool fits = cur.nl + gs <= NL;
-        if (fits && mixed) {
-          const bool have = (cur.mg4[0] && cur.msrc[0] == src) || (cur.mg4[1] && cur.msrc[1] == src);
-          fits = have || cur.mg4[1] == 0;
-        }
-        if (!fits) flush();
-      }
-      if (cur.nl == 0) open_bundle();
-      if (mixed) {
-        if (cur.mg4[0] && cur.msrc[0] == src) slot = 1;
-        else if (cur.mg4[1] && cur.msrc[1] == src) slot = 2;
-        else if (!cur.mg4[0]) {
-          cur.msrc[0] = src;
-          cur.mg4[0] = g4;
-          slot = 1;
-        } else {
-          cur.msrc[1] = src;
-          cur.mg4[1] = g4;
-          slot = 2;
-        }
-      }
-      const size_t bb = heads.size() * (size_t)NL;
+    int slot = 0;
+    if (mixed) {
+      const int src = mat_of[g.u0].first, g4 = mat_of[g.u0].second;
+      const int q = cur.mg4[0] ? 1 : 0;
+      cur.msrc[q] = src;
+      cur.mg4[q] = g4;
+      slot = q + 1;
+    }
+    const size_t bb = heads.size() * (size_t)NL;
+    for (int c = 0; c < count; c++) {
+      const int r = next_r[gi]++;
       uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
       for (int k = 0; k < gs; k++) {
         const int col = g.cols[k];
@@ -835,8 +817,43 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
       cur.nl += gs;
       cur.nu += 1;
     }
+  };
+  // Bundles: the largest remaining group's units, as many as fit, then the
+  // free lines filled with the units of the largest remaining group that fits
+  // (one matrix each, so at most two per bundle).  Units of one group keep
+  // their r order; the line order in HBM is whatever the groups' columns are.
+  std::vector<int> order(S.groups.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return S.groups[a].cols.size() > S.groups[b].cols.size(); });
+  auto gsize = [&](int oi) { return (int)S.groups[order[oi]].cols.size(); };
+  size_t head_pos = 0;
+  while (true) {
+    while (head_pos < order.size() && next_r[order[head_pos]] >= n2) head_pos++;
+    if (head_pos == order.size()) break;
+    const int big = order[head_pos], gb = gsize((int)head_pos);
+    loff.resize(loff.size() + NL, 0);
+    llam.resize(llam.size() + NL, 0);
+    uni.resize(uni.size() + NL, 0);
+    add_units(big, std::min(n2 - next_r[big], NL / gb));
+    const int free_lines = NL - cur.nl;
+    if (free_lines > 0) {
+      // first group in size order with at most free_lines lines and units left
+      size_t lo = head_pos, hi = order.size();
+      while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (gsize((int)mid) > free_lines) lo = mid + 1;
+        else hi = mid;
+      }
+      for (size_t oi = lo; oi < order.size(); oi++) {
+        const int gi = order[oi];
+        if (gi == big || next_r[gi] >= n2) continue;
+        add_units(gi, std::min(n2 - next_r[gi], free_lines / gsize((int)oi)));
+        break;
+      }
+    }
+    flush();
   }
-  flush();
   const size_t lds = lds_fixed + 2 * (size_t)maxu * sizeof(float);
   if (lds + (size_t)NL * 12 > 160 * 1024) return QD_OK;
   QD_RC(upload(&h->mx_head, heads.data(), heads.size()));
