@@ -71,13 +71,16 @@ def _run(prop, steps, warmup, profile):
 
 
 def timed_single(qd, prob, precision, pipeline, steps, warmup, profile=0, autotune=0):
+    # whole flux blocks replayed as hipGraphs (the profiled runs launch eagerly,
+    # with events between the kernels)
     with qd.Propagator(prob.copy(), precision=precision, pipeline=pipeline, profile=profile,
-                       autotune=autotune) as prop:
+                       autotune=autotune, use_graph=0 if profile else 1) as prop:
         return _run(prop, steps, warmup, profile)
 
 
 def timed_batched(qd, jk, precision, steps, warmup, profile=0, autotune=0):
-    with qd.propagator.JKPropagator(jk, precision=precision, profile=profile, autotune=autotune) as prop:
+    with qd.propagator.JKPropagator(jk, precision=precision, profile=profile, autotune=autotune,
+                                    use_graph=0 if profile else 1) as prop:
         return _run(prop, steps, warmup, profile)
 
 
