@@ -508,8 +508,9 @@ constexpr size_t rpass_mx_lds_fixed() {  // without the unit matrices (2 * maxu 
 
 // acc_n = sum_k M[k][n] x_k over the G x G matrix M (row stride G, zero
 // padded beyond the unit's g lines) for the lines line0 .. line0 + g - 1 of
-// the staging rows at point p; out(n, acc_n) for n < g.  The loops have no
-// branches, so the compiler issues LDS reads ahead of the FMAs.
+// the staging rows at point p; out(n, acc_n) for n < g.  The k loop runs over
+// the g lines (g is uniform over the wave); its body has no branches, so the
+// compiler issues the LDS reads ahead of the FMAs.
 template <int G, int SP, int NL, typename OUT>
 __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, int p, const float *M, OUT out) {
   float2 acc[G];
@@ -519,9 +520,8 @@ __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, in
   // flight to cover their latency without holding the whole matrix in registers
   constexpr int UK = G > 32 ? 1 : 2;
 #pragma unroll UK
-  for (int k = 0; k < G; k++) {
-    const float2 xr = stg[min(line0 + k, NL - 1) * SP + p];
-    const float2 x = make_float2(k < g ? xr.x : 0.f, k < g ? xr.y : 0.f);
+  for (int k = 0; k < g; k++) {
+    const float2 x = stg[(line0 + k) * SP + p];
 #pragma unroll
     for (int n4 = 0; n4 < G / 4; n4++) {
       const float4 w = *reinterpret_cast<const float4 *>(M + k * G + 4 * n4);
