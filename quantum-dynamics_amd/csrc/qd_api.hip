@@ -31,6 +31,7 @@
 #include "qd_rfft.hpp"
 #include "qd_kernels.hpp"
 #include "qd_dispatch.hpp"
+#include "qd_pack.hpp"
 #include "qd_tables.hpp"
 
 using namespace qd;
@@ -791,9 +792,8 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
     maxu = std::max(maxu, cur.mg4[0] * cur.mg4[0] + cur.mg4[1] * cur.mg4[1]);
     cur = MxHead{};
   };
-  // units (group, r) of group gi, as many as fit, into the open bundle
-  std::vector<int> next_r(S.groups.size(), 0);
-  auto add_units = [&](int gi, int count) {
+  // units r0 .. r0 + count - 1 of group gi into the open bundle
+  auto add_units = [&](int gi, int r0, int count) {
     const Group &g = S.groups[gi];
     const int gs = (int)g.cols.size();
     const bool mixed = g.u0 >= 0 && gs > 1;
@@ -806,8 +806,7 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
       slot = q + 1;
     }
     const size_t bb = heads.size() * (size_t)NL;
-    for (int c = 0; c < count; c++) {
-      const int r = next_r[gi]++;
+    for (int r = r0; r < r0 + count; r++) {
       uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
       for (int k = 0; k < gs; k++) {
         const int col = g.cols[k];
@@ -818,40 +817,15 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
       cur.nu += 1;
     }
   };
-  // Bundles: the largest remaining group's units, as many as fit, then the
-  // free lines filled with the units of the largest remaining group that fits
-  // (one matrix each, so at most two per bundle).  Units of one group keep
-  // their r order; the line order in HBM is whatever the groups' columns are.
-  std::vector<int> order(S.groups.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return S.groups[a].cols.size() > S.groups[b].cols.size(); });
-  auto gsize = [&](int oi) { return (int)S.groups[order[oi]].cols.size(); };
-  size_t head_pos = 0;
-  while (true) {
-    while (head_pos < order.size() && next_r[order[head_pos]] >= n2) head_pos++;
-    if (head_pos == order.size()) break;
-    const int big = order[head_pos], gb = gsize((int)head_pos);
+  std::vector<int> gsize(S.groups.size());
+  for (size_t i = 0; i < S.groups.size(); i++) gsize[i] = (int)S.groups[i].cols.size();
+  std::vector<std::vector<qdp::Run>> plan;
+  if (!qdp::pack_bundles(gsize, n2, NL, plan)) return QD_OK;
+  for (const auto &runs : plan) {
     loff.resize(loff.size() + NL, 0);
     llam.resize(llam.size() + NL, 0);
     uni.resize(uni.size() + NL, 0);
-    add_units(big, std::min(n2 - next_r[big], NL / gb));
-    const int free_lines = NL - cur.nl;
-    if (free_lines > 0) {
-      // first group in size order with at most free_lines lines and units left
-      size_t lo = head_pos, hi = order.size();
-      while (lo < hi) {
-        const size_t mid = (lo + hi) / 2;
-        if (gsize((int)mid) > free_lines) lo = mid + 1;
-        else hi = mid;
-      }
-      for (size_t oi = lo; oi < order.size(); oi++) {
-        const int gi = order[oi];
-        if (gi == big || next_r[gi] >= n2) continue;
-        add_units(gi, std::min(n2 - next_r[gi], free_lines / gsize((int)oi)));
-        break;
-      }
-    }
+    for (const qdp::Run &run : runs) add_units(run.group, run.r0, run.count);
     flush();
   }
   const size_t lds = lds_fixed + 2 * (size_t)maxu * sizeof(float);

@@ -58,3 +58,29 @@ def test_centrifugal_eta_shift(qd):
     assert np.allclose(jk.eta_sq_J(J), want, rtol=1e-12, atol=0)
     # the shifted |eta|^2 is the plain one evaluated shift lower in energy: its peak moves up by the shift
     assert jk.base.energies[np.argmax(jk.eta_sq_J(J))] >= jk.base.energies[np.argmax(jk.base.eta_sq)]
+
+
+@pytest.fixture(scope="module")
+def pack_harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("pack") / "pack")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "cpu_harness", "pack_main.cpp")],
+                   check=True)
+    return exe
+
+
+@pytest.mark.parametrize("Jmax,jmax,n2,nl,min_fill", [
+    (20, 60, 192, 32, 0.99),   # the batched workload: 231 blocks, groups of up to 21 lines
+    (40, 60, 192, 48, 0.90),   # every J up to 40: groups of up to 41 lines in 48-line bundles
+    (9, 11, 16, 32, 0.0),      # a small batch (fewer units than bundles can pair)
+])
+def test_bundle_packing(pack_harness, Jmax, jmax, n2, nl, min_fill):
+    """Every mixing unit lands in exactly one bundle of at most nl lines and two
+    groups, in r order within its group; the largest-first packing with a
+    complementary partner fills the batched workload's line slots completely."""
+    out = subprocess.run([pack_harness, str(Jmax), str(jmax), str(n2), str(nl)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    assert out[0] == "ok", " ".join(out)
+    nb, lines, fill = int(out[1]), int(out[2]), float(out[3])
+    groups = sum(min(J, j) + 1 for J in range(Jmax + 1) for j in range(jmax + 1))
+    assert lines == groups * n2
+    assert fill >= min_fill, fill
