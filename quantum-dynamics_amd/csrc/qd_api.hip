@@ -183,6 +183,10 @@ struct qd_handle {
   double *dcol_w = nullptr;
   int *fcol_lam = nullptr, *fcol_j = nullptr;
   BlockDesc *d_blocks = nullptr;
+  // the same descriptors ordered by K (potential set), then J: the matrix-stationary
+  // theta kernels take their blocks in this order, so the blocks that read one
+  // exp(-i V dt) set run together and share it through the caches
+  BlockDesc *d_blocks_k = nullptr;
   double *d1 = nullptr, *d2 = nullptr;
   float *d1f = nullptr, *d2f = nullptr;
   void *dR = nullptr, *dr = nullptr, *kinR = nullptr, *kinr = nullptr, *twR = nullptr, *twr = nullptr;
@@ -412,6 +416,7 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   P.n1 = h->n1;
   if (h->tmx_ks) {
     P.tpw = h->tmx_tpw;
+    P.blocks = h->d_blocks_k;
     const void *fn = theta_mx_kernel(h->tmx_ks, pro, epi);
     if (!fn) return fail(QD_ERR_ARG, "bad theta variant");
     const long long ntiles = (h->n12 + 15) / 16;
@@ -460,7 +465,7 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
   F.Bb = h->Wb;
   F.Bf = h->Wf;
   F.mat_stride = h->mat_stride;
-  F.blocks = h->d_blocks;
+  F.blocks = h->d_blocks_k;
   F.R = (int)h->n12;
   F.expV = h->expV32;
   F.dcol_vidx = h->dcol_vidx;
@@ -1083,6 +1088,11 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     h->rl_slots = std::max(1, nb) * std::max(1, ncu);
   }
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
+  {
+    std::vector<BlockDesc> byk = h->hblocks;
+    std::stable_sort(byk.begin(), byk.end(), [](const BlockDesc &a, const BlockDesc &b) { return a.K < b.K; });
+    QD_RC(upload(&h->d_blocks_k, byk.data(), byk.size()));
+  }
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));
   QD_RC(upload(&h->dcol_w, dcol_w.data(), dcol_w.size()));
   QD_RC(upload(&h->bundle_u0, bu0.data(), bu0.size()));
@@ -1492,7 +1502,7 @@ int spec_from_jk(const qd_jk_problem *q, BatchSpec &S) {
 void release(qd_handle *h) {
   if (!h) return;
   void *bufs[] = {h->psi64,    h->lpsi64, h->psi32,     h->lpsi32,   h->V,      h->expV32,   h->dcol_vidx,
-                  h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d1,     h->d2,       h->d1f,
+                  h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d_blocks_k, h->d1,   h->d2,       h->d1f,
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
