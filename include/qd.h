@@ -206,8 +206,13 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
  * [9] its unit-size bucket, [10] fused theta step in use, [11] its 16-row
- * tiles per work group.  vals holds 12 entries. */
+ * tiles per work group, [12] that mixing R pass runs two work groups per CU.
+ * vals holds 13 entries. */
 int qd_get_tuning(qd_handle *h, int32_t *vals);
+/* test hook: force one kernel choice the autotuner makes.  what 0: the
+ * prefetching FP32 mixing R pass with one (value 0) or two (value 1) work
+ * groups per CU; QD_ERR_ARG when the handle does not have that kernel. */
+int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);
 

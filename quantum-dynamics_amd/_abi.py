@@ -132,13 +132,14 @@ def load_library(path: str = LIB_PATH):
     lib.qd_jk_theta_grid.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
     lib.qd_get_crp_blocks.argtypes = [C.c_void_p, _dp]
     lib.qd_debug_stage.argtypes = [C.c_void_p, C.c_int32]
+    lib.qd_debug_select.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.qd_debug_get_fbr.argtypes = [C.c_void_p, _dp]
     lib.qd_debug_set_fbr.argtypes = [C.c_void_p, _dp]
     lib.qd_last_error.restype = C.c_char_p
     for f in ("qd_time_evolution", "qd_create", "qd_propagate", "qd_set_callback", "qd_get_psi",
               "qd_get_crp", "qd_get_norm", "qd_get_steps", "qd_get_timing", "qd_destroy", "qd_set_psi",
               "qd_create_jk", "qd_jk_theta_grid", "qd_get_crp_blocks", "qd_debug_stage", "qd_debug_get_fbr",
-              "qd_debug_set_fbr"):
+              "qd_debug_set_fbr", "qd_debug_select"):
         getattr(lib, f).restype = C.c_int
     _lib = lib
     return lib

@@ -208,8 +208,9 @@ struct qd_handle {
   int *bundle_l0 = nullptr, *line_lam = nullptr;  // mixing R pass: per-bundle line tables
   // FP32 mixing R pass with prefetch (k_rpass_mx): bundle records
   bool rp_mx = false, rp_legacy = true;  // legacy: the LDS-staged and Stockham R passes are set up
-  int mx_nl = 0, mx_gb = 0, mx_maxu = 0, mx_nbundle = 0;
-  long long mx_slots = 1;
+  int mx_nl = 0, mx_gb = 0, mx_nbundle = 0;
+  bool mx_two = false;                   // the variant sized for two work groups per CU
+  long long mx_slots = 1, mx_slots2 = 0;  // resident work groups of each variant (0: none)
   size_t mx_lds = 0;
   MxHead *mx_head = nullptr;
   long long *mx_off = nullptr;
@@ -322,11 +323,10 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
       m.unit = h->mx_unit;
       m.umat = h->mx_umat;
       m.nbundle = h->mx_nbundle;
-      m.maxu = h->mx_maxu;
       size_t lds0 = 0;
-      const void *fn = rpass_mx_kernel(h->n1, h->mx_nl, h->mx_gb, &lds0);
+      const void *fn = rpass_mx_kernel(h->n1, h->mx_nl, h->mx_gb, h->mx_two, &lds0);
       void *args[] = {&m};
-      const unsigned nblk = (unsigned)std::min(h->mx_slots, (long long)h->mx_nbundle);
+      const unsigned nblk = (unsigned)std::min(h->mx_two ? h->mx_slots2 : h->mx_slots, (long long)h->mx_nbundle);
       QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->mx_nl * rf_n1(h->n1)), args, h->mx_lds, h->stream));
       QD_HIP(hipGetLastError());
       return QD_OK;
@@ -766,8 +766,8 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
   const int n1 = h->n1, n2 = h->n2;
   const int NL = gmax <= 32 ? 32 : 48;
   const int GB = gmax <= 8 ? 8 : gmax <= 16 ? 16 : gmax <= 24 ? 24 : gmax <= 32 ? 32 : 48;
-  size_t lds_fixed = 0;
-  const void *fn = rpass_mx_kernel(n1, NL, GB, &lds_fixed);
+  size_t lds = 0;
+  const void *fn = rpass_mx_kernel(n1, NL, GB, false, &lds);
   if (!fn) return QD_OK;
   std::vector<float> upad;
   std::map<int, std::pair<int, int>> mat_of;  // group u0 -> (offset, g4)
@@ -790,11 +790,9 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
   std::vector<long long> loff;
   std::vector<int> llam, uni;
   MxHead cur{};
-  int maxu = 0;
   auto flush = [&]() {
     if (cur.nl == 0) return;
     heads.push_back(cur);
-    maxu = std::max(maxu, cur.mg4[0] * cur.mg4[0] + cur.mg4[1] * cur.mg4[1]);
     cur = MxHead{};
   };
   // units r0 .. r0 + count - 1 of group gi into the open bundle
@@ -833,7 +831,6 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
     for (const qdp::Run &run : runs) add_units(run.group, run.r0, run.count);
     flush();
   }
-  const size_t lds = lds_fixed + 2 * (size_t)maxu * sizeof(float);
   if (lds + (size_t)NL * 12 > 160 * 1024) return QD_OK;
   QD_RC(upload(&h->mx_head, heads.data(), heads.size()));
   QD_RC(upload(&h->mx_off, loff.data(), loff.size()));
@@ -846,12 +843,22 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
   QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nthreads, lds));
   QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
   if (nb < 1) return QD_OK;
+  // the variant for two work groups per CU, where both fit
+  size_t lds2 = 0;
+  const void *fn2 = rpass_mx_kernel(n1, NL, GB, true, &lds2);
+  h->mx_slots2 = 0;
+  if (fn2 && lds2 == lds) {
+    int nb2 = 0;
+    QD_HIP(hipFuncSetAttribute(fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, fn2, nthreads, lds));
+    if (nb2 >= 2) h->mx_slots2 = (long long)nb2 * std::max(1, ncu);
+  }
   h->mx_nl = NL;
   h->mx_gb = GB;
-  h->mx_maxu = std::max(maxu, 4);
   h->mx_lds = lds;
   h->mx_nbundle = (int)heads.size();
   h->mx_slots = (long long)nb * std::max(1, ncu);
+  h->mx_two = h->mx_slots2 > 0;  // measured faster where it fits; the autotuner re-checks
   h->rp_mx = true;
   return QD_OK;
 }
@@ -1721,19 +1728,22 @@ int autotune_handle(qd_handle *h) {
     *flag = t_rf <= t_st;
     return QD_OK;
   };
-  // R pass: the prefetching mixing kernel, the register FFT with LDS-staged
-  // mixing, or the Stockham kernels, whichever of them this handle has
+  // R pass: the prefetching mixing kernel (one or two work groups per CU),
+  // the register FFT with LDS-staged mixing, or the Stockham kernels,
+  // whichever of them this handle has
   {
-    const bool has_mx = h->rp_mx, has_rf = h->rp_rf, has_st = h->rp_legacy;
+    const bool has_mx = h->rp_mx, has_mx2 = h->rp_mx && h->mx_slots2 > 0, has_rf = h->rp_rf,
+               has_st = h->rp_legacy;
     auto runR = [&] {
       return f64 ? launch_rpass<double>(h, full, (double2 *)lpsi) : launch_rpass<float>(h, full, (float2 *)lpsi);
     };
     float best = 1e30f;
     int pick = has_mx ? 0 : has_rf ? 1 : 2;
-    for (int c = 0; c < 3; c++) {
-      if ((c == 0 && !has_mx) || (c == 1 && !has_rf) || (c == 2 && !has_st)) continue;
+    for (int c = 0; c < 4; c++) {
+      if ((c == 0 && !has_mx) || (c == 1 && !has_rf) || (c == 2 && !has_st) || (c == 3 && !has_mx2)) continue;
       if (c == 2 && !has_mx && !has_rf) continue;  // the only choice
-      h->rp_mx = c == 0;
+      h->rp_mx = c == 0 || c == 3;
+      h->mx_two = c == 3;
       h->rp_rf = c == 1;
       float ms = 0;
       QD_RC(time_launches(h, runR, reps, &ms));
@@ -1742,7 +1752,8 @@ int autotune_handle(qd_handle *h) {
         pick = c;
       }
     }
-    h->rp_mx = pick == 0;
+    h->rp_mx = pick == 0 || pick == 3;
+    h->mx_two = pick == 3;
     h->rp_rf = pick == 1;
   }
   QD_RC(pick_pass(&h->rl_rf, false));
@@ -1801,6 +1812,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[9] = h->rp_mx ? h->mx_gb : 0;
   vals[10] = h->fuse_theta ? 1 : 0;
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
+  vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
   return QD_OK;
 }
 
@@ -1946,6 +1958,20 @@ int qd_debug_set_fbr(qd_handle *h, const double *in) {
     QD_HIP(hipFree(tmp));
   }
   return QD_OK;
+}
+
+int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
+  if (!h) return fail(QD_ERR_NULL, "handle is NULL");
+  if (what == 0) {
+    if (h->mx_nbundle == 0) return fail(QD_ERR_ARG, "this handle has no prefetching mixing R pass");
+    if (value == 1 && h->mx_slots2 == 0) return fail(QD_ERR_ARG, "two work groups per CU do not fit this handle");
+    if (value != 0 && value != 1) return fail(QD_ERR_ARG, "value must be 0 or 1");
+    h->rp_mx = true;
+    h->rp_rf = false;
+    h->mx_two = value == 1;
+    return QD_OK;
+  }
+  return fail(QD_ERR_ARG, "unknown selection");
 }
 
 int qd_debug_stage(qd_handle *h, int32_t stage) {

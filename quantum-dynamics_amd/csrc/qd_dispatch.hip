@@ -199,33 +199,35 @@ const void *theta_fused_kernel(int ks, size_t *lds) {
 
 #if QD_PART == 5 || QD_PART == 6
 template <int N1, int N2, int NL, int GB>
-const void *rpass_mx_fn(size_t *lds) {
-  *lds = rpass_mx_lds_fixed<N1, N2, NL>();
-  return (const void *)k_rpass_mx<N1, N2, NL, GB>;
+const void *rpass_mx_fn(bool two, size_t *lds) {
+  *lds = rpass_mx_lds<N1, N2, NL>();
+  if constexpr (NL == 32)
+    if (two) return (const void *)k_rpass_mx<N1, N2, NL, GB, true>;
+  return (const void *)k_rpass_mx<N1, N2, NL, GB, false>;
 }
 template <int N1, int N2>
-const void *rpass_mx_len(int nl, int gb, size_t *lds) {
+const void *rpass_mx_len(int nl, int gb, bool two, size_t *lds) {
   if (nl == 32) {
-    if (gb == 8) return rpass_mx_fn<N1, N2, 32, 8>(lds);
-    if (gb == 16) return rpass_mx_fn<N1, N2, 32, 16>(lds);
-    if (gb == 24) return rpass_mx_fn<N1, N2, 32, 24>(lds);
-    if (gb == 32) return rpass_mx_fn<N1, N2, 32, 32>(lds);
-  } else if (nl == 48 && gb == 48) {
-    return rpass_mx_fn<N1, N2, 48, 48>(lds);
+    if (gb == 8) return rpass_mx_fn<N1, N2, 32, 8>(two, lds);
+    if (gb == 16) return rpass_mx_fn<N1, N2, 32, 16>(two, lds);
+    if (gb == 24) return rpass_mx_fn<N1, N2, 32, 24>(two, lds);
+    if (gb == 32) return rpass_mx_fn<N1, N2, 32, 32>(two, lds);
+  } else if (nl == 48 && gb == 48 && !two) {
+    return rpass_mx_fn<N1, N2, 48, 48>(false, lds);
   }
   return nullptr;
 }
 #endif
 #if QD_PART == 5
-const void *rpass_mx_kernel(int n, int nl, int gb, size_t *lds_fixed) {
-  if (n == 256) return rpass_mx_len<16, 16>(nl, gb, lds_fixed);
-  return rpass_mx_kernel_small(n, nl, gb, lds_fixed);
+const void *rpass_mx_kernel(int n, int nl, int gb, bool two, size_t *lds) {
+  if (n == 256) return rpass_mx_len<16, 16>(nl, gb, two, lds);
+  return rpass_mx_kernel_small(n, nl, gb, two, lds);
 }
 #endif
 #if QD_PART == 6
-const void *rpass_mx_kernel_small(int n, int nl, int gb, size_t *lds_fixed) {
-  if (n == 64) return rpass_mx_len<8, 8>(nl, gb, lds_fixed);
-  if (n == 128) return rpass_mx_len<16, 8>(nl, gb, lds_fixed);
+const void *rpass_mx_kernel_small(int n, int nl, int gb, bool two, size_t *lds) {
+  if (n == 64) return rpass_mx_len<8, 8>(nl, gb, two, lds);
+  if (n == 128) return rpass_mx_len<16, 8>(nl, gb, two, lds);
   return nullptr;
 }
 #endif

@@ -26,10 +26,11 @@ const void *rpass_rf_mix_kernel(int n, int lw, size_t *lds);
 template <typename T>
 const void *rline_rf_kernel(int n, size_t *lds, int *lpb);
 // FP32 mixing R pass with prefetch (k_rpass_mx) for line length n, nl lines
-// per bundle and unit sizes up to gb; *lds_fixed = dynamic LDS bytes before
-// the unit matrices.  nullptr when no instantiation exists.
-const void *rpass_mx_kernel(int n, int nl, int gb, size_t *lds_fixed);
-const void *rpass_mx_kernel_small(int n, int nl, int gb, size_t *lds_fixed);
+// per bundle and unit sizes up to gb, sized for one or (two) two resident work
+// groups per CU; *lds = its dynamic LDS bytes.  nullptr when no instantiation
+// exists.
+const void *rpass_mx_kernel(int n, int nl, int gb, bool two, size_t *lds);
+const void *rpass_mx_kernel_small(int n, int nl, int gb, bool two, size_t *lds);
 // matrix-stationary mixed theta GEMM for ks k-steps, V-phase prologue pro, dump epilogue epi
 // (epi 2: FBR -> DVR times conj(exp(-i V dt)), the DVR state from the fused step's FBR state)
 const void *theta_mx_kernel(int ks, int pro, int epi);

@@ -468,12 +468,13 @@ constexpr size_t rpass_rf_lds() {
 // per row; the row doubles as that line's FFT exchange buffer.  Per bundle:
 //   mix K -> l      one (unit, point) per thread, the unit's g outputs
 //                   accumulated in registers and written in place (only this
-//                   thread touches this point of the unit's lines);
+//                   thread touches this point of the unit's lines); the
+//                   unit matrix comes from HBM/L2 through scalar loads;
 //   FFT phase       line L: D_R/2, forward FFT, kinetic, inverse FFT, D_R/2;
 //   mix l -> K      read the g values of the point, store the K lines
 //                   straight to HBM.
 // The next bundle's descriptors are loaded during the K -> l mixing and its
-// lines and matrices during the FFT and the l -> K mixing, so the HBM traffic
+// lines during the FFT and the l -> K mixing, so the HBM traffic
 // of one bundle overlaps the arithmetic of the previous one.
 struct MxHead {
   int nl, nu;    // lines and units of the bundle
@@ -493,7 +494,6 @@ struct RMixArgs {
   const int *unit;           // [nbundle][NL] line0 | g << 8 | slot << 16 (slot 0: no mixing)
   const float *umat;         // padded U and U^T of every unit matrix
   int nbundle;
-  int maxu;                  // LDS entries per direction for the bundle's matrices
 };
 
 template <int N1, int N2, int NL>
@@ -502,29 +502,36 @@ constexpr int mx_row_stride() {
   return LX > SPA ? LX : SPA;
 }
 template <int N1, int N2, int NL>
-constexpr size_t rpass_mx_lds_fixed() {  // without the unit matrices (2 * maxu floats)
+constexpr size_t rpass_mx_lds() {
   return rf_table_bytes<N1, N2>(sizeof(float4)) + (size_t)NL * mx_row_stride<N1, N2, NL>() * sizeof(float2);
 }
+
+// Unit matrices are read through the constant address space: their addresses
+// are uniform over a wave, so they become scalar loads into SGPRs that feed
+// the FMAs directly, and the LDS bandwidth stays with the line values.
+typedef float qd_f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) float kfloat;
+typedef const __attribute__((address_space(4))) qd_f4v kfloat4;
 
 // acc_n = sum_k M[k][n] x_k over the G x G matrix M (row stride G, zero
 // padded beyond the unit's g lines) for the lines line0 .. line0 + g - 1 of
 // the staging rows at point p; out(n, acc_n) for n < g.  The k loop runs over
 // the g lines (g is uniform over the wave); its body has no branches, so the
-// compiler issues the LDS reads ahead of the FMAs.
+// compiler issues the loads ahead of the FMAs.
 template <int G, int SP, int NL, typename OUT>
-__device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, int p, const float *M, OUT out) {
+__device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, int p, kfloat *M, OUT out) {
   float2 acc[G];
 #pragma unroll
   for (int n = 0; n < G; n++) acc[n] = make_float2(0.f, 0.f);
-  // two k-steps per iteration (one above 32 lines): enough LDS reads in
-  // flight to cover their latency without holding the whole matrix in registers
-  constexpr int UK = G > 32 ? 1 : 2;
+  // two k-steps per iteration up to 24 lines: enough loads in flight to cover
+  // their latency within the scalar register budget
+  constexpr int UK = G > 24 ? 1 : 2;
 #pragma unroll UK
   for (int k = 0; k < g; k++) {
     const float2 x = stg[(line0 + k) * SP + p];
 #pragma unroll
     for (int n4 = 0; n4 < G / 4; n4++) {
-      const float4 w = *reinterpret_cast<const float4 *>(M + k * G + 4 * n4);
+      const qd_f4v w = *reinterpret_cast<kfloat4 *>(M + k * G + 4 * n4);
       acc[4 * n4].x = fmaf(x.x, w.x, acc[4 * n4].x);
       acc[4 * n4].y = fmaf(x.y, w.x, acc[4 * n4].y);
       acc[4 * n4 + 1].x = fmaf(x.x, w.y, acc[4 * n4 + 1].x);
@@ -541,7 +548,7 @@ __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, in
 }
 // mx_apply for the unit's padded order 4 ceil(g / 4) (at most GB)
 template <int GB, int SP, int NL, typename OUT>
-__device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int p, const float *M, OUT out) {
+__device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int p, kfloat *M, OUT out) {
 #define QD_MXA(G_) mx_apply<G_, SP, NL>(stg, line0, g, p, M, out)
   switch ((g + 3) >> 2) {
     case 1: QD_MXA(4); break;
@@ -560,22 +567,21 @@ __device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int
 #undef QD_MXA
 }
 
-template <int N1, int N2, int NL, int GB>
-__global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
+// TWO: sized for two resident work groups per CU (at most 128 registers per
+// thread, LDS within half the CU's): the lines are loaded at the top of their
+// own iteration, and the other work group's bundle covers the latency.
+template <int N1, int N2, int NL, int GB, bool TWO>
+__global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(RMixArgs A) {
   typedef float2 C;
   typedef float4 TB;
   constexpr int N = N1 * N2, NT = NL * N1, XS = N2 + 1;
   constexpr int SP = mx_row_stride<N1, N2, NL>();
-  constexpr int UMAX = 2 * GB * GB;                  // bound on the matrix entries of a bundle
-  constexpr int UPT = (UMAX + NT - 1) / NT;          // prefetched matrix entries per thread and direction
   static_assert(N % 64 == 0 && NT % N == 0, "the points of a unit must fill whole waves");
   static_assert(GB % 4 == 0 && GB <= NL && NL <= 64, "unit size bucket");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TB *s_twa = reinterpret_cast<TB *>(smem_raw);
   TB *s_kin = s_twa + N1 * XS;
   C *stg = reinterpret_cast<C *>(s_kin + N);          // [NL][SP]
-  float *s_U = reinterpret_cast<float *>(stg + NL * SP);
-  float *s_UT = s_U + A.maxu;
   __shared__ long long s_off[NL];
   __shared__ int s_unit[NL];
   const int t = threadIdx.x, a = t % N1, L = t / N1;
@@ -598,9 +604,8 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
       h1 = make_int4(0, 0, 0, 0);
     }
   };
-  // lines and matrices of a bundle (registers), from its descriptors
+  // lines of a bundle (registers), from its descriptors
   C xn[N2];
-  float un[UPT], unt[UPT];
   auto fetch_data = [&]() {
     const int nl = h0.x;
     if (L < nl) {
@@ -608,24 +613,12 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
 #pragma unroll
       for (int m = 0; m < N2; m++) xn[m] = src[a + N1 * m];
     }
-    const int g0 = h1.x, g1 = h1.y;     // mg4[0], mg4[1]
-    const int n0 = g0 * g0, nu_ = n0 + g1 * g1;
-#pragma unroll
-    for (int q = 0; q < UPT; q++) {
-      const int i = t + q * NT;
-      if (i < nu_) {
-        const int slot = i >= n0, ii = slot ? i - n0 : i, gg = slot ? g1 : g0;
-        const float *src = A.umat + (slot ? h0.w : h0.z);
-        un[q] = src[ii];
-        unt[q] = src[gg * gg + ii];
-      }
-    }
   };
 
   // With 48-line bundles (768 threads, at most 168 registers per thread) the
   // lines are loaded at the top of their own iteration instead of one bundle
   // ahead, which keeps the 48 mixing accumulators in registers.
-  constexpr bool PREF = NT <= 512;
+  constexpr bool PREF = NT <= 512 && !TWO;
   int b = blockIdx.x;
   if (b < A.nbundle) {
     fetch_desc(b, true);
@@ -635,6 +628,7 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
     if (!PREF) fetch_data();
     const int nl = __builtin_amdgcn_readfirstlane(h0.x), nu = __builtin_amdgcn_readfirstlane(h0.y);
     const int g40 = __builtin_amdgcn_readfirstlane(h1.x), g41 = __builtin_amdgcn_readfirstlane(h1.y);
+    const int ms0 = __builtin_amdgcn_readfirstlane(h0.z), ms1 = __builtin_amdgcn_readfirstlane(h0.w);
     const int lam = d_lam;
     __syncthreads();  // the previous bundle is done with LDS (first time: tables)
     if (L < nl) {
@@ -643,17 +637,6 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
       if (a == 0) s_off[L] = d_off;
     }
     if (t < NL) s_unit[t] = d_unit;
-    {
-      const int nuu = g40 * g40 + g41 * g41;
-#pragma unroll
-      for (int q = 0; q < UPT; q++) {
-        const int i = t + q * NT;
-        if (i < nuu) {
-          s_U[i] = un[q];
-          s_UT[i] = unt[q];
-        }
-      }
-    }
     const int bn = b + (int)gridDim.x;
     fetch_desc(bn, bn < A.nbundle);
     __syncthreads();
@@ -666,13 +649,13 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
         const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
         const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
         if (slot) {
-          const float *U = s_U + (slot == 1 ? 0 : g40 * g40);
+          kfloat *U = (kfloat *)(A.umat + (slot == 1 ? ms0 : ms1));
           mx_unit<GB, SP, NL>(stg, line0, g, p, U, [&](int m, C c) { stg[(line0 + m) * SP + p] = c; });
         }
       }
     }
     __syncthreads();
-    // the next bundle's lines and matrices stream in during the rest
+    // the next bundle's lines stream in during the rest
     if (PREF) fetch_data();
     // FFT phase: line L in the l basis
     {
@@ -705,7 +688,8 @@ __global__ __launch_bounds__(NL * N1) void k_rpass_mx(RMixArgs A) {
         const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
         const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
         if (slot) {
-          const float *UT = s_UT + (slot == 1 ? 0 : g40 * g40);
+          const int g4 = slot == 1 ? g40 : g41;
+          kfloat *UT = (kfloat *)(A.umat + (slot == 1 ? ms0 : ms1) + g4 * g4);
           mx_unit<GB, SP, NL>(stg, line0, g, p, UT, [&](int K, C c) { A.data[s_off[line0 + K] + p] = c; });
         } else {
           for (int K = 0; K < g; K++) A.data[s_off[line0 + K] + p] = stg[(line0 + K) * SP + p];

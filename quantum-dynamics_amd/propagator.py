@@ -172,12 +172,12 @@ class Propagator:
 
     def tuning(self) -> dict:
         """Kernel configuration in use (qd_get_tuning)."""
-        v = (C.c_int32 * 12)()
+        v = (C.c_int32 * 13)()
         _abi.check(self.lib.qd_get_tuning(self.h, v))
         keys = ("autotuned", "theta_ksteps", "theta_tiles_per_group", "rpass_register_fft",
                 "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group",
                 "rpass_prefetch_mix_bundle_lines", "rpass_prefetch_mix_group_bucket", "fused_theta",
-                "fused_theta_tiles_per_group")
+                "fused_theta_tiles_per_group", "rpass_prefetch_mix_two_groups")
         return dict(zip(keys, list(v)))
 
     def timing(self, which: int) -> float:
@@ -232,7 +232,11 @@ def _debug_methods(cls):
         assert a.size == 2 * self.n_points
         _abi.check(self.lib.qd_set_psi(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
 
+    def debug_select(self, what: int, value: int):
+        _abi.check(self.lib.qd_debug_select(self.h, int(what), int(value)))
+
     cls.debug_stage = debug_stage
+    cls.debug_select = debug_select
     cls.get_fbr = get_fbr
     cls.set_fbr = set_fbr
     cls.set_psi = set_psi

@@ -82,23 +82,26 @@ def test_jk_norm_conserved_without_absorbers(qd, jkcase, prec):
 
 def test_jk_prefetch_mixing_mixed_within_1e4_of_fp64(qd):
     """Mixed-precision batch on a 64-point R grid (prefetching FP32 mixing R
-    pass) against the FP64 batch over 1500 steps: per-energy |dP^J(E)| within
-    1e-4, and the norm of a no-absorber run within 1e-5 of its start."""
+    pass, with one and with two work groups per CU) against the FP64 batch
+    over 1500 steps: per-energy |dP^J(E)| within 1e-4, and the norm of a
+    no-absorber run within 1e-5 of its start."""
     pb = qd.problem
     base = pb.config(0, nR=64, nr=32, ntheta=12, steps=1500, e0_ev=0.6)
     jk = pb.make_jk_problem(base, pb.H_H2, [0, 4, 9], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
     print("case: jk mixed vs fp64, prefetch mixing, 1500 steps", flush=True)
     out = {}
-    for prec in ("fp64", "mixed"):
-        with qd.propagator.JKPropagator(jk, precision=prec) as prop:
-            if prec == "mixed":
+    for prec in ("fp64", "mixed", "mixed_two"):
+        with qd.propagator.JKPropagator(jk, precision=prec[:5] if prec != "fp64" else prec) as prop:
+            if prec != "fp64":
                 assert prop.tuning()["rpass_prefetch_mix_bundle_lines"] == 32
+                prop.debug_select(0, 1 if prec == "mixed_two" else 0)
             prop.propagate(base.total_steps)
             out[prec] = prop.crp_by_J()
-    worst = max(float(np.max(np.abs(out["mixed"][J] - out["fp64"][J]))) for J in out["fp64"])
-    print("max |dP^J(E)| mixed vs fp64: %.2e" % worst)
     assert max(float(np.max(out["fp64"][J])) for J in out["fp64"]) > 1e-3  # a non-trivial P
-    assert worst < 1e-4
+    for m in ("mixed", "mixed_two"):
+        worst = max(float(np.max(np.abs(out[m][J] - out["fp64"][J]))) for J in out["fp64"])
+        print("max |dP^J(E)| %s vs fp64: %.2e" % (m, worst))
+        assert worst < 1e-4
 
     b2 = base.copy()
     b2.dump1 = None

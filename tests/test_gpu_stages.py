@@ -117,9 +117,10 @@ def test_fused_passes_with_phases(qd, small, prec):
 
 # (R points, r points, J values, j_max): LDS-staged mixing (32 points), the
 # prefetching FP32 kernel with 32-line bundles (64 points, groups up to 10
-# lines) and with 48-line bundles (groups of 34 lines)
+# lines; "two": its variant for two work groups per CU, on 256 points) and
+# with 48-line bundles (groups of 34 lines)
 MIX_CASES = {"staged": (32, 24, [0, 2, 3], 5), "prefetch32": (64, 16, [0, 3, 9], 11),
-             "prefetch48": (64, 16, [33], 35)}
+             "prefetch32two": (256, 16, [0, 3, 9], 11), "prefetch48": (64, 16, [33], 35)}
 
 
 @pytest.mark.parametrize("case", sorted(MIX_CASES))
@@ -141,6 +142,10 @@ def test_r_pass_mixing_jk(qd, prec, case):
     with qd.propagator.JKPropagator(jk, precision=prec) as prop:
         tot = jk.psi.size
         y = rnd(tot, 31)
+        if prec == "mixed" and case.startswith("prefetch"):
+            two = 1 if case.endswith("two") else 0
+            prop.debug_select(0, two)
+            assert prop.tuning()["rpass_prefetch_mix_two_groups"] == two
         prop.set_fbr(as_f64(y))
         prop.debug_stage(1)
         got = prop.get_fbr().view(np.complex128)
