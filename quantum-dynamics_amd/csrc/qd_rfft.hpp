@@ -662,21 +662,30 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
       const bool act = L < nl;
       C *row = stg + L * SP;
       C v[N2];
-      TB phv[N2];
+      TB phv[PREF ? N2 : 1];
       if (act) {
         const TB *ph = A.dR + (size_t)lam * N;
+        if constexpr (PREF) {
 #pragma unroll
-        for (int m = 0; m < N2; m++) phv[m] = ph[a + N1 * m];
-      }
-      if (act) {
+          for (int m = 0; m < N2; m++) phv[m] = ph[a + N1 * m];
 #pragma unroll
-        for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], phv[m], false);
+          for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], phv[m], false);
+        } else {
+          // four rows at a time: the loads are not all hoisted ahead, which
+          // keeps the pass within its register budget without spilling
+#pragma unroll
+          for (int m0 = 0; m0 < N2; m0 += 4) {
+#pragma unroll
+            for (int m = m0; m < m0 + 4; m++) v[m] = tmul(row[a + N1 * m], ph[a + N1 * m], false);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
       }
       rf_line<float, N1, N2>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
       if (act) {
         const TB *ph = A.dR + (size_t)lam * N;
 #pragma unroll
-        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], PREF ? phv[m] : ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], PREF ? phv[PREF ? m : 0] : ph[a + N1 * m], false);
       }
     }
     __syncthreads();

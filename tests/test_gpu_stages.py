@@ -117,10 +117,10 @@ def test_fused_passes_with_phases(qd, small, prec):
 
 # (R points, r points, J values, j_max): LDS-staged mixing (32 points), the
 # prefetching FP32 kernel with 32-line bundles (64 points, groups up to 10
-# lines; "two": its variant for two work groups per CU, on 256 points) and
-# with 48-line bundles (groups of 34 lines)
+# lines; "two": its variant for two work groups per CU, on 256 points with
+# groups of up to 21 lines) and with 48-line bundles (groups of 34 lines)
 MIX_CASES = {"staged": (32, 24, [0, 2, 3], 5), "prefetch32": (64, 16, [0, 3, 9], 11),
-             "prefetch32two": (256, 16, [0, 3, 9], 11), "prefetch48": (64, 16, [33], 35)}
+             "prefetch32two": (256, 16, [0, 4, 20], 22), "prefetch48": (64, 16, [33], 35)}
 
 
 @pytest.mark.parametrize("case", sorted(MIX_CASES))
