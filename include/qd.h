@@ -211,7 +211,8 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
 int qd_get_tuning(qd_handle *h, int32_t *vals);
 /* test hook: force one kernel choice the autotuner makes.  what 0: the
  * prefetching FP32 mixing R pass with one (value 0) or two (value 1) work
- * groups per CU; QD_ERR_ARG when the handle does not have that kernel. */
+ * groups per CU; QD_ERR_ARG when the handle does not have that kernel.
+ * what 1: its consecutive bundles per turn of a work group (1..64, default 8). */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

@@ -210,6 +210,7 @@ struct qd_handle {
   bool rp_mx = false, rp_legacy = true;  // legacy: the LDS-staged and Stockham R passes are set up
   int mx_nl = 0, mx_gb = 0, mx_nbundle = 0;
   bool mx_two = false;                   // the variant sized for two work groups per CU
+  int mx_chunk = MX_CHUNK;               // consecutive bundles per turn of a work group
   long long mx_slots = 1, mx_slots2 = 0;  // resident work groups of each variant (0: none)
   size_t mx_lds = 0;
   MxHead *mx_head = nullptr;
@@ -323,10 +324,12 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
       m.unit = h->mx_unit;
       m.umat = h->mx_umat;
       m.nbundle = h->mx_nbundle;
+      m.chunk = h->mx_chunk;
       size_t lds0 = 0;
       const void *fn = rpass_mx_kernel(h->n1, h->mx_nl, h->mx_gb, h->mx_two, &lds0);
       void *args[] = {&m};
-      const unsigned nblk = (unsigned)std::min(h->mx_two ? h->mx_slots2 : h->mx_slots, (long long)h->mx_nbundle);
+      const unsigned nblk = (unsigned)std::min(h->mx_two ? h->mx_slots2 : h->mx_slots,
+                                               ((long long)h->mx_nbundle + h->mx_chunk - 1) / h->mx_chunk);
       QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->mx_nl * rf_n1(h->n1)), args, h->mx_lds, h->stream));
       QD_HIP(hipGetLastError());
       return QD_OK;
@@ -1969,6 +1972,11 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     h->rp_mx = true;
     h->rp_rf = false;
     h->mx_two = value == 1;
+    return QD_OK;
+  }
+  if (what == 1) {
+    if (value < 1 || value > 64) return fail(QD_ERR_ARG, "chunk must be 1..64");
+    h->mx_chunk = value;
     return QD_OK;
   }
   return fail(QD_ERR_ARG, "unknown selection");

@@ -146,6 +146,8 @@ def test_r_pass_mixing_jk(qd, prec, case):
             two = 1 if case.endswith("two") else 0
             prop.debug_select(0, two)
             assert prop.tuning()["rpass_prefetch_mix_two_groups"] == two
+            if case == "prefetch32":
+                prop.debug_select(1, 3)  # three consecutive bundles per turn of a work group
         prop.set_fbr(as_f64(y))
         prop.debug_stage(1)
         got = prop.get_fbr().view(np.complex128)
