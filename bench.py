@@ -50,7 +50,7 @@ B_ALG = {
 # WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/*.md.  "theta"
 # is the mean of theta-fwd and theta-bwd (the event class averages both).
 MEASURED_B_PER_POINT = {
-    "batched": {"theta": 24.5, "fft_R": 16.5, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
+    "batched": {"theta": 24.4, "fft_R": 16.4, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
     "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
@@ -192,7 +192,7 @@ def roofline(kern, npt, workload, fused_theta, prec="mixed"):
             "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[workload][dom] * npt,
             "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt,
             "note": ("the R pass with K<->l mixing is VALU-bound (about 159 VALU instructions per point: the "
-                     "register FFT plus the mixing in both directions; VALU issue about 60 % of its cycles), not "
+                     "register FFT plus the mixing in both directions; VALU issue about 67 % of its cycles), not "
                      "HBM-bound; it moves its 16 algorithmic bytes per point" if (dom == "fft_R" and workload == "batched")
                      else "")}
 
