@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("batched", "single", "jinterp", "accuracy"), default="batched",
                     help="jinterp: the learned J interpolation end to end (F+H2, every 4th J of 0..40 propagated, "
-                         "the network trained on the GPU, judged against propagating all 41 J)")
+                         "the network trained on the GPU, checked against propagating all 41 J)")
     ap.add_argument("--jinterp-steps", type=int, default=10000, help="propagation steps of the jinterp workload")
     ap.add_argument("--out", help="jinterp: also write the full report (per-energy sums) as JSON here")
     ap.add_argument("--calibrate", action="store_true",

@@ -11,7 +11,7 @@ skipped J from three features:
 where B_J = J(J+1) / (2 mu R_b^2) is the centrifugal barrier at a
 representative atom-diatom distance R_b.  The method without the network is
 linear interpolation of P^J(E) in J at fixed E (`linear_in_J`); the
-propagator itself judges both on held-out J.
+propagator itself checks both on held-out J.
 """
 from __future__ import annotations
 
@@ -246,7 +246,7 @@ def learned_partial_waves(qd, base, system, J_max: int = 40, stride: int = 4, n_
     """The learned J interpolation end to end.
 
     Propagates every `stride`-th J of 0..J_max, trains the network on their
-    P^J(E) on the GPU, predicts the skipped J, and judges the result with the
+    P^J(E) on the GPU, predicts the skipped J, and checks the result with the
     propagator itself: `reference` = "all" propagates every J (the partial-wave
     sum without the network and its wall time), "held" only `n_held` skipped J
     spread over the range.  Reports the per-energy error of the partial-wave
