@@ -224,6 +224,11 @@ struct qd_handle {
   bool fuse_theta = false, state_fbr = false, dvr_valid = true;
   int fuse_opt = 0, tfu_tpw = 1;
   size_t tfu_lds = 0;
+  // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
+  // matrices and the V columns, and the work items built from them
+  std::vector<std::pair<int, int>> tfu_groups;  // first block, blocks
+  int4 *d_tfu_items = nullptr;
+  int tfu_nitems = 0, tfu_items_cap = 0;
 
   bool rp_ct = false, rl_ct = false;
   size_t rp_lds = 0;
@@ -477,15 +482,39 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
   F.n1 = h->n1;
   F.r_lo = h->do_crp && rows ? h->surf - h->stencil.h : 1;
   F.r_hi = h->do_crp && rows ? h->surf + h->stencil.h : 0;
-  F.tpw = h->tfu_tpw;
+  F.items = h->d_tfu_items;
   size_t lds = 0;
   const void *fn = theta_fused_kernel(h->tmx_ks, &lds);
   if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
-  const long long ntiles = (h->n12 + 15) / 16;
-  dim3 grid((unsigned)((ntiles + h->tfu_tpw - 1) / h->tfu_tpw), 1, h->nb);
+  if (h->tfu_nitems < 1) return fail(QD_ERR_ARG, "fused theta work items not set up");
+  dim3 grid((unsigned)h->tfu_nitems, 1, 1);
   void *args[] = {&F};
   QD_HIP(hipLaunchKernel(fn, grid, dim3(TMX_THREADS), args, lds, h->stream));
   QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+// Work items of the fused theta kernel for the current tfu_tpw: each group's
+// tiles in ranges of max(8, tfu_tpw / blocks) tiles, so that every item holds
+// about tfu_tpw tile-blocks.  Set-up and autotuning only (a synchronous copy).
+int build_tfu_items(qd_handle *h) {
+  const long long W = TMX_THREADS / 64, ntiles = ((long long)h->n12 + 15) / 16;
+  std::vector<int4> items;
+  for (const auto &g : h->tfu_groups) {
+    const long long tc = std::min(ntiles, std::max(W, ((long long)h->tfu_tpw + g.second - 1) / g.second));
+    for (long long t = 0; t < ntiles; t += tc)
+      items.push_back(make_int4(g.first, g.second, (int)t, (int)std::min(ntiles, t + tc)));
+  }
+  if ((int)items.size() > h->tfu_items_cap) {
+    if (h->d_tfu_items) QD_HIP(hipFree(h->d_tfu_items));
+    h->d_tfu_items = nullptr;
+    h->tfu_items_cap = 0;
+    QD_HIP(hipMalloc((void **)&h->d_tfu_items, items.size() * sizeof(int4)));
+    h->tfu_items_cap = (int)items.size();
+  }
+  QD_HIP(hipStreamSynchronize(h->stream));
+  QD_HIP(hipMemcpy(h->d_tfu_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  h->tfu_nitems = (int)items.size();
   return QD_OK;
 }
 
@@ -1102,6 +1131,16 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     std::vector<BlockDesc> byk = h->hblocks;
     std::stable_sort(byk.begin(), byk.end(), [](const BlockDesc &a, const BlockDesc &b) { return a.K < b.K; });
     QD_RC(upload(&h->d_blocks_k, byk.data(), byk.size()));
+    // runs of blocks sharing the theta matrices, the basis sizes and the V columns
+    h->tfu_groups.clear();
+    for (size_t i = 0; i < byk.size();) {
+      size_t j = i + 1;
+      while (j < byk.size() && byk[j].mat == byk[i].mat && byk[j].nd == byk[i].nd && byk[j].nf == byk[i].nf &&
+             dcol_vidx[byk[j].dcol0] == dcol_vidx[byk[i].dcol0])
+        j++;
+      h->tfu_groups.push_back({(int)i, (int)(j - i)});
+      i = j;
+    }
   }
   QD_RC(upload(&h->dcol_vidx, dcol_vidx.data(), dcol_vidx.size()));
   QD_RC(upload(&h->dcol_w, dcol_w.data(), dcol_w.size()));
@@ -1336,6 +1375,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
         h->fuse_theta = true;
         h->tfu_lds = flds;
         h->tfu_tpw = h->tmx_tpw;
+        QD_RC(build_tfu_items(h));
       }
     }
   }
@@ -1517,7 +1557,7 @@ void release(qd_handle *h) {
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mx_head,  h->mx_off,
-                  h->mx_lam,   h->mx_unit, h->mx_umat};
+                  h->mx_lam,   h->mx_unit, h->mx_umat, h->d_tfu_items};
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (void *b : bufs)
@@ -1707,6 +1747,7 @@ int autotune_handle(qd_handle *h) {
     for (int c : cands) {
       if (c < 1 || c > ntiles) continue;
       h->tfu_tpw = c;
+      QD_RC(build_tfu_items(h));
       float ms = 0;
       QD_RC(time_launches(h, [&] { return launch_theta_fused(h, false); }, reps, &ms));
       if (ms < best) {
@@ -1715,6 +1756,7 @@ int autotune_handle(qd_handle *h) {
       }
     }
     h->tfu_tpw = pick;
+    QD_RC(build_tfu_items(h));
   }
   // R and r passes: register FFT or Stockham kernels
   auto pick_pass = [&](bool *flag, bool alongR) -> int {

@@ -1127,6 +1127,10 @@ __global__ __launch_bounds__(512) void k_theta_mx(ThetaArgs P) {
 // into each MFMA k-block, and that grouping was measured to gain norm
 // coherently, 6e-8 per step; the natural grouping of the unfused kernels
 // does not.)
+// Blocks that share the theta matrices and the potential (the J blocks of one
+// K) form a group.  A work group takes one work item: a range of tiles of one
+// group; for each tile a wave loads the V phase once and runs every block of
+// the group through it, so V is read from HBM once per group, not per block.
 struct ThetaFusedArgs {
   float2 *L;                   // FBR array, in and out
   float2 *psi;                 // DVR array: the flux rows are written
@@ -1139,7 +1143,7 @@ struct ThetaFusedArgs {
   const float *d1f, *d2f;      // dump factors (nullptr: no dump)
   int n1;
   int r_lo, r_hi;              // flux rows (r_lo > r_hi: none)
-  int tpw;
+  const int4 *items;           // per work group: first block of its group, blocks, tile begin, tile end
 };
 
 template <int KS>
@@ -1150,10 +1154,12 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
   __bf16 *Mf = Mb + 3 * NP * LD;                     // [3][NP][LD], row n = FBR column, k = DVR column
   // per wave, the tile's DVR values [16][XS] in natural column order
   float2 *Xw = reinterpret_cast<float2 *>(Mf + 3 * NP * LD) + (threadIdx.x >> 6) * 16 * XS;
-  const BlockDesc bd = P.blocks[blockIdx.z];
+  const int4 item = P.items[blockIdx.x];
+  const int gb0 = item.x, gnb = item.y, t0 = item.z, t1 = item.w;
+  const BlockDesc bd = P.blocks[gb0];  // the group shares mat, nd, nf and the V columns
   const int R = P.R, nd = bd.nd, nf = bd.nf;
-  const __amdgpu_buffer_rsrc_t rL = buf_rsrc(P.L + (size_t)bd.fcol0 * R, (unsigned)nf * R * 8u);
-  const __amdgpu_buffer_rsrc_t rP = buf_rsrc(P.psi + (size_t)bd.dcol0 * R, (unsigned)nd * R * 8u);
+  auto rL_of = [&](int b) { return buf_rsrc(P.L + (size_t)P.blocks[gb0 + b].fcol0 * R, (unsigned)nf * R * 8u); };
+  auto rP_of = [&](int b) { return buf_rsrc(P.psi + (size_t)P.blocks[gb0 + b].dcol0 * R, (unsigned)nd * R * 8u); };
   const __amdgpu_buffer_rsrc_t rV = buf_rsrc(P.expV + P.dcol_vidx[bd.dcol0], (unsigned)nd * R * 8u);
   {
     const __bf16 *Gb = reinterpret_cast<const __bf16 *>(P.Bb) + (size_t)bd.mat * P.mat_stride;
@@ -1169,15 +1175,13 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
   const int rl = lane & 15, kg = lane >> 4;
-  const int ntiles = (R + 15) >> 4;
-  const int t1 = min(ntiles, (int)(blockIdx.x + 1) * P.tpw);
   float2 ra[8];
-  auto fetch = [&](int tile, int s) {
+  auto fetch = [&](const __amdgpu_buffer_rsrc_t &rd, int tile, int s) {
     const int row = tile * 16 + rl;
     const int k0 = s * 32 + kg * 8;
     const unsigned base = (row < R ? (unsigned)row * 8u : TM_OOB) + (unsigned)(R * k0) * 8u;
 #pragma unroll
-    for (int e = 0; e < 8; e++) ra[e] = buf_ld_f2(rL, k0 + e < nf ? base + (unsigned)(R * e) * 8u : TM_OOB);
+    for (int e = 0; e < 8; e++) ra[e] = buf_ld_f2(rd, k0 + e < nf ? base + (unsigned)(R * e) * 8u : TM_OOB);
   };
   // three BF16 pieces of 8 FP32 values
   auto split3 = [](const float *x, bf8v *out) {
@@ -1193,20 +1197,29 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
         r[e] -= (float)b;
       }
   };
-  int tile = blockIdx.x * P.tpw + wave;
-  if (tile < t1) fetch(tile, 0);
-  for (; tile < t1; tile += W) {
+  // this wave's (tile, block) sequence: tiles t0 + wave, + W, ...; every
+  // block of the group for each tile
+  int tile = t0 + wave, bi = 0;
+  __amdgpu_buffer_rsrc_t rL = rL_of(0);
+  if (tile < t1) fetch(rL, tile, 0);
+  float2 wv[NT][4];
+  while (tile < t1) {
     const int row = tile * 16 + rl;
     const bool rin = row < R;
-    // V phase of this tile's DVR columns, loaded ahead of the backward GEMM
-    float2 wv[NT][4];
+    const bool lastb = bi + 1 == gnb;
+    const int ntile = lastb ? tile + W : tile, nbi = lastb ? 0 : bi + 1;
+    const __amdgpu_buffer_rsrc_t rLn = rL_of(nbi);
+    // V phase of this tile's DVR columns, loaded once per tile ahead of the
+    // first block's backward GEMM
+    if (bi == 0) {
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++)
+      for (int nt = 0; nt < NT; nt++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int n = nt * 16 + 4 * kg + v;
-        wv[nt][v] = buf_ld_f2(rV, (rin && n < nd) ? (unsigned)(row + R * n) * 8u : TM_OOB);
-      }
+        for (int v = 0; v < 4; v++) {
+          const int n = nt * 16 + 4 * kg + v;
+          wv[nt][v] = buf_ld_f2(rV, (rin && n < nd) ? (unsigned)(row + R * n) * 8u : TM_OOB);
+        }
+    }
     f4v acc[2][NT];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1226,11 +1239,8 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
         split3(re, bre);
         split3(im, bim);
       }
-      {
-        const bool last = s + 1 == KS;
-        const int ft = last ? tile + W : tile;
-        if (ft < t1) fetch(ft, last ? 0 : s + 1);
-      }
+      if (s + 1 < KS) fetch(rL, tile, s + 1);
+      else if (ntile < t1) fetch(rLn, ntile, 0);
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) {
         bf8v a[3];
@@ -1250,6 +1260,7 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
     if (P.d1f && rin) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
     const int rr = row / P.n1;
     const bool frow = rin && rr >= P.r_lo && rr <= P.r_hi;
+    const __amdgpu_buffer_rsrc_t rP = rP_of(bi);
 #pragma unroll
     for (int nt = 0; nt < NT; nt++)
 #pragma unroll
@@ -1309,6 +1320,9 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
         const int m = nt * 16 + 4 * kg + v;
         buf_st_f2(make_float2(acc[0][nt][v], acc[1][nt][v]), rL, (rin && m < nf) ? (unsigned)(row + R * m) * 8u : TM_OOB);
       }
+    tile = ntile;
+    bi = nbi;
+    rL = rLn;
   }
 }
 template <int KS>
