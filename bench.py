@@ -41,7 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # and r passes and theta-bwd each read and write one complex array.
 # With the fused theta step (mixed precision, basis of at most 64) one kernel
 # per step reads the FBR array and exp(-i V dt) (8 B each) and writes the FBR
-# array: 24 B per point.
+# array: 24 B per point of a single block.  Blocks that share V (the J blocks
+# of one K) read it once per step: 16 B + 8 B x (V columns / DVR columns), 16.7 B
+# per point on the J = 0..20 batch (1071 V columns, 12551 DVR columns).
 B_ALG = {
     "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "theta_fused": 24, "fft_R": 16, "fft_r": 16, "total": 68},
     "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
@@ -50,7 +52,7 @@ B_ALG = {
 # WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/*.md.  "theta"
 # is the mean of theta-fwd and theta-bwd (the event class averages both).
 MEASURED_B_PER_POINT = {
-    "batched": {"theta": 24.4, "fft_R": 16.4, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
+    "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
     "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
