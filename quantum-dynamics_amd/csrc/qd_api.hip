@@ -140,6 +140,17 @@ struct BatchSpec {
 
 }  // namespace
 
+// One class of bundles of the FP32 mixing R pass (k_rpass_mx): its kernel
+// shape and its device records.
+struct MxSet {
+  int nl = 0, gb = 0, nbundle = 0;
+  long long slots = 0, slots2 = 0;  // resident work groups with one / two per CU (0: does not fit)
+  size_t lds = 0;
+  MxHead *head = nullptr;
+  long long *off = nullptr;
+  int *lam = nullptr, *unit = nullptr;
+};
+
 struct qd_handle {
   // configuration
   int dev = 0;
@@ -208,15 +219,11 @@ struct qd_handle {
   int *bundle_l0 = nullptr, *line_lam = nullptr;  // mixing R pass: per-bundle line tables
   // FP32 mixing R pass with prefetch (k_rpass_mx): bundle records
   bool rp_mx = false, rp_legacy = true;  // legacy: the LDS-staged and Stockham R passes are set up
-  int mx_nl = 0, mx_gb = 0, mx_nbundle = 0;
-  bool mx_two = false;                   // the variant sized for two work groups per CU
+  // [0]: 32-line bundles (groups of at most 32 lines), [1]: 48-line bundles
+  MxSet mxs[2];
+  bool mx_two = false;                   // 32-line class: the variant sized for two work groups per CU
   int mx_chunk = MX_CHUNK;               // consecutive bundles per turn of a work group
-  long long mx_slots = 1, mx_slots2 = 0;  // resident work groups of each variant (0: none)
-  size_t mx_lds = 0;
-  MxHead *mx_head = nullptr;
-  long long *mx_off = nullptr;
-  int *mx_lam = nullptr, *mx_unit = nullptr;
-  float *mx_umat = nullptr;
+  float *mx_umat = nullptr;              // unit matrices of both classes
   long long *line_off = nullptr;
   size_t tmx_lds = 0;
   // fused theta step (k_theta_fused): between steps the state is the FBR array
@@ -323,20 +330,27 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
       m.tw = (const float4 *)h->twR;
       m.kin = (const float4 *)h->kinR;
       m.dR = (const float4 *)h->dR;
-      m.head = h->mx_head;
-      m.line_off = h->mx_off;
-      m.line_lam = h->mx_lam;
-      m.unit = h->mx_unit;
       m.umat = h->mx_umat;
-      m.nbundle = h->mx_nbundle;
       m.chunk = h->mx_chunk;
-      size_t lds0 = 0;
-      const void *fn = rpass_mx_kernel(h->n1, h->mx_nl, h->mx_gb, h->mx_two, &lds0);
-      void *args[] = {&m};
-      const unsigned nblk = (unsigned)std::min(h->mx_two ? h->mx_slots2 : h->mx_slots,
-                                               ((long long)h->mx_nbundle + h->mx_chunk - 1) / h->mx_chunk);
-      QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->mx_nl * rf_n1(h->n1)), args, h->mx_lds, h->stream));
-      QD_HIP(hipGetLastError());
+      // the two bundle classes touch disjoint lines
+      for (int c = 0; c < 2; c++) {
+        const MxSet &M = h->mxs[c];
+        if (M.nbundle == 0) continue;
+        const bool two = c == 0 && h->mx_two && M.slots2 > 0;
+        m.head = M.head;
+        m.line_off = M.off;
+        m.line_lam = M.lam;
+        m.unit = M.unit;
+        m.nbundle = M.nbundle;
+        size_t lds0 = 0;
+        const void *fn = rpass_mx_kernel(h->n1, M.nl, M.gb, two, &lds0);
+        if (!fn) return fail(QD_ERR_ARG, "no mixing R-pass kernel for this bundle class");
+        void *args[] = {&m};
+        const unsigned nblk =
+            (unsigned)std::min(two ? M.slots2 : M.slots, ((long long)M.nbundle + h->mx_chunk - 1) / h->mx_chunk);
+        QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(M.nl * rf_n1(h->n1)), args, M.lds, h->stream));
+        QD_HIP(hipGetLastError());
+      }
       return QD_OK;
     }
   }
@@ -790,17 +804,17 @@ int setup_theta_mx(qd_handle *h) {
 }
 
 // Bundle records of the FP32 mixing R pass (k_rpass_mx): units (group, r)
-// packed in order into bundles of at most NL lines and two distinct unit
-// matrices; per bundle a header and NL-strided line and unit entries.  The
-// matrices are stored padded to g4 = 4 ceil(g / 4), U then U^T, rounded to
-// FP32 with unit column lengths (qdt::round_orthogonal_columns).
+// packed into bundles of at most NL lines and two distinct unit matrices
+// (qdp::pack_bundles); per bundle a header and NL-strided line and unit
+// entries.  Groups of at most 32 lines go to 32-line bundles, larger groups to
+// 48-line bundles: two classes, each with its own kernel, so that a batch with
+// a few large groups still runs most of its lines on the 32-line kernel.  The
+// matrices (shared by both classes) are stored padded to g4 = 4 ceil(g / 4),
+// U then U^T, rounded to FP32 with unit column lengths
+// (qdt::round_orthogonal_columns).
 int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam, int gmax) {
+  (void)gmax;
   const int n1 = h->n1, n2 = h->n2;
-  const int NL = gmax <= 32 ? 32 : 48;
-  const int GB = gmax <= 8 ? 8 : gmax <= 16 ? 16 : gmax <= 24 ? 24 : gmax <= 32 ? 32 : 48;
-  size_t lds = 0;
-  const void *fn = rpass_mx_kernel(n1, NL, GB, false, &lds);
-  if (!fn) return QD_OK;
   std::vector<float> upad;
   std::map<int, std::pair<int, int>> mat_of;  // group u0 -> (offset, g4)
   for (const Group &g : S.groups) {
@@ -818,79 +832,99 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
       }
     mat_of[g.u0] = {off, g4};
   }
-  std::vector<MxHead> heads;
-  std::vector<long long> loff;
-  std::vector<int> llam, uni;
-  MxHead cur{};
-  auto flush = [&]() {
-    if (cur.nl == 0) return;
-    heads.push_back(cur);
-    cur = MxHead{};
-  };
-  // units r0 .. r0 + count - 1 of group gi into the open bundle
-  auto add_units = [&](int gi, int r0, int count) {
-    const Group &g = S.groups[gi];
-    const int gs = (int)g.cols.size();
-    const bool mixed = g.u0 >= 0 && gs > 1;
-    int slot = 0;
-    if (mixed) {
-      const int src = mat_of[g.u0].first, g4 = mat_of[g.u0].second;
-      const int q = cur.mg4[0] ? 1 : 0;
-      cur.msrc[q] = src;
-      cur.mg4[q] = g4;
-      slot = q + 1;
-    }
-    const size_t bb = heads.size() * (size_t)NL;
-    for (int r = r0; r < r0 + count; r++) {
-      uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
-      for (int k = 0; k < gs; k++) {
-        const int col = g.cols[k];
-        loff[bb + cur.nl + k] = ((long long)col * n2 + r) * n1;
-        llam[bb + cur.nl + k] = fcol_lam[col];
-      }
-      cur.nl += gs;
-      cur.nu += 1;
-    }
-  };
-  std::vector<int> gsize(S.groups.size());
-  for (size_t i = 0; i < S.groups.size(); i++) gsize[i] = (int)S.groups[i].cols.size();
-  std::vector<std::vector<qdp::Run>> plan;
-  if (!qdp::pack_bundles(gsize, n2, NL, plan)) return QD_OK;
-  for (const auto &runs : plan) {
-    loff.resize(loff.size() + NL, 0);
-    llam.resize(llam.size() + NL, 0);
-    uni.resize(uni.size() + NL, 0);
-    for (const qdp::Run &run : runs) add_units(run.group, run.r0, run.count);
-    flush();
-  }
-  if (lds + (size_t)NL * 12 > 160 * 1024) return QD_OK;
-  QD_RC(upload(&h->mx_head, heads.data(), heads.size()));
-  QD_RC(upload(&h->mx_off, loff.data(), loff.size()));
-  QD_RC(upload(&h->mx_lam, llam.data(), llam.size()));
-  QD_RC(upload(&h->mx_unit, uni.data(), uni.size()));
-  QD_RC(upload(&h->mx_umat, upad.data(), upad.size()));
-  QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int nthreads = NL * rf_n1(n1);
-  int nb = 0, ncu = 0;
-  QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nthreads, lds));
+  int ncu = 0;
   QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
-  if (nb < 1) return QD_OK;
-  // the variant for two work groups per CU, where both fit
-  size_t lds2 = 0;
-  const void *fn2 = rpass_mx_kernel(n1, NL, GB, true, &lds2);
-  h->mx_slots2 = 0;
-  if (fn2 && lds2 == lds) {
-    int nb2 = 0;
-    QD_HIP(hipFuncSetAttribute(fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, fn2, nthreads, lds));
-    if (nb2 >= 2) h->mx_slots2 = (long long)nb2 * std::max(1, ncu);
+  struct HostSet {
+    int nl = 0, gb = 0;
+    size_t lds = 0;
+    const void *fn = nullptr;
+    std::vector<MxHead> heads;
+    std::vector<long long> loff;
+    std::vector<int> llam, uni;
+  } hs[2];
+  std::vector<int> sel[2];
+  for (size_t i = 0; i < S.groups.size(); i++) sel[S.groups[i].cols.size() > 32 ? 1 : 0].push_back((int)i);
+  for (int c = 0; c < 2; c++) {
+    if (sel[c].empty()) continue;
+    HostSet &H = hs[c];
+    int gm = 1;
+    for (int gi : sel[c]) gm = std::max(gm, (int)S.groups[gi].cols.size());
+    H.nl = c == 0 ? 32 : 48;
+    H.gb = gm <= 8 ? 8 : gm <= 16 ? 16 : gm <= 24 ? 24 : gm <= 32 ? 32 : 48;
+    H.fn = rpass_mx_kernel(n1, H.nl, H.gb, false, &H.lds);
+    if (!H.fn || H.lds + (size_t)H.nl * 12 > 160 * 1024) return QD_OK;  // no mixing kernel for this batch
+    const int NL = H.nl;
+    MxHead cur{};
+    // units r0 .. r0 + count - 1 of group gi into the open bundle
+    auto add_units = [&](int gi, int r0, int count) {
+      const Group &g = S.groups[gi];
+      const int gs = (int)g.cols.size();
+      const bool mixed = g.u0 >= 0 && gs > 1;
+      int slot = 0;
+      if (mixed) {
+        const int src = mat_of[g.u0].first, g4 = mat_of[g.u0].second;
+        const int q = cur.mg4[0] ? 1 : 0;
+        cur.msrc[q] = src;
+        cur.mg4[q] = g4;
+        slot = q + 1;
+      }
+      const size_t bb = H.heads.size() * (size_t)NL;
+      for (int r = r0; r < r0 + count; r++) {
+        H.uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
+        for (int k = 0; k < gs; k++) {
+          const int col = g.cols[k];
+          H.loff[bb + cur.nl + k] = ((long long)col * n2 + r) * n1;
+          H.llam[bb + cur.nl + k] = fcol_lam[col];
+        }
+        cur.nl += gs;
+        cur.nu += 1;
+      }
+    };
+    std::vector<int> gsize(sel[c].size());
+    for (size_t i = 0; i < sel[c].size(); i++) gsize[i] = (int)S.groups[sel[c][i]].cols.size();
+    std::vector<std::vector<qdp::Run>> plan;
+    if (!qdp::pack_bundles(gsize, n2, NL, plan)) return QD_OK;
+    for (const auto &runs : plan) {
+      H.loff.resize(H.loff.size() + NL, 0);
+      H.llam.resize(H.llam.size() + NL, 0);
+      H.uni.resize(H.uni.size() + NL, 0);
+      for (const qdp::Run &run : runs) add_units(sel[c][run.group], run.r0, run.count);
+      if (cur.nl) H.heads.push_back(cur);
+      cur = MxHead{};
+    }
   }
-  h->mx_nl = NL;
-  h->mx_gb = GB;
-  h->mx_lds = lds;
-  h->mx_nbundle = (int)heads.size();
-  h->mx_slots = (long long)nb * std::max(1, ncu);
-  h->mx_two = h->mx_slots2 > 0;  // measured faster where it fits; the autotuner re-checks
+  // both classes fit: upload and size the launches
+  QD_RC(upload(&h->mx_umat, upad.data(), upad.size()));
+  for (int c = 0; c < 2; c++) {
+    const HostSet &H = hs[c];
+    MxSet &M = h->mxs[c];
+    if (H.heads.empty()) continue;
+    QD_RC(upload(&M.head, H.heads.data(), H.heads.size()));
+    QD_RC(upload(&M.off, H.loff.data(), H.loff.size()));
+    QD_RC(upload(&M.lam, H.llam.data(), H.llam.size()));
+    QD_RC(upload(&M.unit, H.uni.data(), H.uni.size()));
+    const int nthreads = H.nl * rf_n1(n1);
+    int nb = 0;
+    QD_HIP(hipFuncSetAttribute(H.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)H.lds));
+    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, H.fn, nthreads, H.lds));
+    if (nb < 1) return fail(QD_ERR_ARG, "the mixing R pass does not fit a CU");
+    M.slots = (long long)nb * std::max(1, ncu);
+    // the variant for two work groups per CU, where both fit
+    M.slots2 = 0;
+    size_t lds2 = 0;
+    const void *fn2 = rpass_mx_kernel(n1, H.nl, H.gb, true, &lds2);
+    if (fn2 && lds2 == H.lds) {
+      int nb2 = 0;
+      QD_HIP(hipFuncSetAttribute(fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)H.lds));
+      QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, fn2, nthreads, H.lds));
+      if (nb2 >= 2) M.slots2 = (long long)nb2 * std::max(1, ncu);
+    }
+    M.nl = H.nl;
+    M.gb = H.gb;
+    M.lds = H.lds;
+    M.nbundle = (int)H.heads.size();
+  }
+  h->mx_two = h->mxs[0].slots2 > 0;  // measured faster where it fits; the autotuner re-checks
   h->rp_mx = true;
   return QD_OK;
 }
@@ -1556,8 +1590,9 @@ void release(qd_handle *h) {
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
-                  h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mx_head,  h->mx_off,
-                  h->mx_lam,   h->mx_unit, h->mx_umat, h->d_tfu_items};
+                  h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
+                  h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
+                  h->mx_umat,  h->d_tfu_items};
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (void *b : bufs)
@@ -1777,7 +1812,7 @@ int autotune_handle(qd_handle *h) {
   // the register FFT with LDS-staged mixing, or the Stockham kernels,
   // whichever of them this handle has
   {
-    const bool has_mx = h->rp_mx, has_mx2 = h->rp_mx && h->mx_slots2 > 0, has_rf = h->rp_rf,
+    const bool has_mx = h->rp_mx, has_mx2 = h->rp_mx && h->mxs[0].slots2 > 0, has_rf = h->rp_rf,
                has_st = h->rp_legacy;
     auto runR = [&] {
       return f64 ? launch_rpass<double>(h, full, (double2 *)lpsi) : launch_rpass<float>(h, full, (float2 *)lpsi);
@@ -1853,8 +1888,8 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[5] = h->rp_lw;
   vals[6] = h->rp_ml;
   vals[7] = h->rl_lpb;
-  vals[8] = h->rp_mx ? h->mx_nl : 0;
-  vals[9] = h->rp_mx ? h->mx_gb : 0;
+  vals[8] = h->rp_mx ? std::max(h->mxs[0].nl, h->mxs[1].nl) : 0;
+  vals[9] = h->rp_mx ? std::max(h->mxs[0].gb, h->mxs[1].gb) : 0;
   vals[10] = h->fuse_theta ? 1 : 0;
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
   vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
@@ -2008,8 +2043,9 @@ int qd_debug_set_fbr(qd_handle *h, const double *in) {
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
   if (!h) return fail(QD_ERR_NULL, "handle is NULL");
   if (what == 0) {
-    if (h->mx_nbundle == 0) return fail(QD_ERR_ARG, "this handle has no prefetching mixing R pass");
-    if (value == 1 && h->mx_slots2 == 0) return fail(QD_ERR_ARG, "two work groups per CU do not fit this handle");
+    if (h->mxs[0].nbundle == 0 && h->mxs[1].nbundle == 0)
+      return fail(QD_ERR_ARG, "this handle has no prefetching mixing R pass");
+    if (value == 1 && h->mxs[0].slots2 == 0) return fail(QD_ERR_ARG, "two work groups per CU do not fit this handle");
     if (value != 0 && value != 1) return fail(QD_ERR_ARG, "value must be 0 or 1");
     h->rp_mx = true;
     h->rp_rf = false;
