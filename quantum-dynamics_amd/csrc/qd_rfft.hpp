@@ -473,9 +473,10 @@ constexpr size_t rpass_rf_lds() {
 //   FFT phase       line L: D_R/2, forward FFT, kinetic, inverse FFT, D_R/2;
 //   mix l -> K      read the g values of the point, store the K lines
 //                   straight to HBM.
-// The next bundle's descriptors are loaded during the K -> l mixing and its
-// lines during the FFT and the l -> K mixing, so the HBM traffic
-// of one bundle overlaps the arithmetic of the previous one.
+// The next bundle's descriptors are loaded during the K -> l mixing.  With one
+// work group per CU its lines are loaded during the FFT and the l -> K mixing,
+// so the HBM traffic of one bundle overlaps the arithmetic of the previous
+// one; with two work groups per CU (TWO) the other work group covers it.
 struct MxHead {
   int nl, nu;    // lines and units of the bundle
   int msrc[2];   // offsets in umat of its matrices (U, then U^T at + g4^2)
