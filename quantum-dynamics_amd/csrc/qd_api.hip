@@ -243,6 +243,7 @@ struct qd_handle {
   int rl_lpb = 16, rl_blocks = 0;
   size_t rl_lds = 0;
   double2 *bpsi = nullptr, *bdpsi = nullptr, *fai = nullptr, *dfai = nullptr;
+  double2 *ftau = nullptr;  // tau[t][e] of the current time block (k_flux_tau)
   double *energies = nullptr, *eta_sq = nullptr, *crp = nullptr, *partial = nullptr;
   int npartial = 256;
   // timing
@@ -551,9 +552,12 @@ int materialize_psi(qd_handle *h) {
 int flush_flux(qd_handle *h) {
   if (!h->do_crp || h->slot == 0) return QD_OK;
   QD_RC(mark(h, TK_FLUX));
-  dim3 grid((unsigned)((h->nx + FX_X - 1) / FX_X), (unsigned)((h->ne + FX_E - 1) / FX_E));
-  hipLaunchKernelGGL(k_flux_accumulate, grid, dim3(256), 0, h->stream, h->bpsi, h->bdpsi, h->fai, h->dfai,
-                     h->energies, (int)h->nx, h->ne, h->slot, h->d_acc_steps, h->dt);
+  hipLaunchKernelGGL(k_flux_tau, grid1((size_t)h->slot * h->ne), dim3(256), 0, h->stream, h->ftau, h->energies, h->ne,
+                     h->slot, h->d_acc_steps, h->dt);
+  const long long nxb = (h->nx + FX_X - 1) / FX_X, ney = (h->ne + FX_E - 1) / FX_E;
+  dim3 grid((unsigned)(((nxb + 7) / 8) * 8 * ney));
+  hipLaunchKernelGGL(k_flux_accumulate, grid, dim3(256), 0, h->stream, h->bpsi, h->bdpsi, h->fai, h->dfai, h->ftau,
+                     (int)h->nx, h->ne, h->slot);
   hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, h->stream, h->d_acc_steps, h->slot);
   QD_HIP(hipGetLastError());
   h->acc_steps += h->slot;
@@ -1425,6 +1429,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     }
     QD_HIP(hipMalloc((void **)&h->bpsi, sizeof(double2) * h->nx * h->tblock));
     QD_HIP(hipMalloc((void **)&h->bdpsi, sizeof(double2) * h->nx * h->tblock));
+    QD_HIP(hipMalloc((void **)&h->ftau, sizeof(double2) * h->ne * h->tblock));
     QD_HIP(hipMalloc((void **)&h->fai, sizeof(double2) * h->nx * h->ne));
     QD_HIP(hipMalloc((void **)&h->dfai, sizeof(double2) * h->nx * h->ne));
     QD_HIP(hipMemsetAsync(h->fai, 0, sizeof(double2) * h->nx * h->ne, h->stream));
@@ -1588,7 +1593,7 @@ void release(qd_handle *h) {
   void *bufs[] = {h->psi64,    h->lpsi64, h->psi32,     h->lpsi32,   h->V,      h->expV32,   h->dcol_vidx,
                   h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d_blocks_k, h->d1,   h->d2,       h->d1f,
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
-                  h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi,
+                  h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi, h->ftau,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,

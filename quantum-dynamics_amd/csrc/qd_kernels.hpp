@@ -1374,12 +1374,18 @@ __global__ void k_flux_extract(const typename cx_of<T>::type *__restrict__ psi, 
 // the block):  fai[x, e] += sum_t tau[t, e] psi_s[t, x],  same for d fai,
 // tau[t, e] = dt exp(i E_e (n0 + t + 1) dt)  (the reference's recursion
 // timeEvol.C:799-817 evaluated directly, as evolutionCUDA2.cu:143-145 does).
-// Work group: 64 x-values x 32 energies; each thread holds 8 energies.
+// k_flux_tau writes the block's tau table once; k_flux_accumulate stages it
+// and the block's surface values through LDS.  Work group: 128 x-values x 32
+// energies; each thread holds 2 x-values (xl, xl + 64) x 8 energies, so every
+// tau value read from LDS feeds two complex products per array; each complex
+// multiply-add is four FMAs.  Per (x, e) the sum runs over t in increasing
+// order.  The work groups of one x range (its energy blocks) get linear ids
+// 8 apart, so they run on one XCD and read the surface values from its L2.
 // ---------------------------------------------------------------------------
-constexpr int FX_X = 64;
+constexpr int FX_X = 128;
 constexpr int FX_E = 32;
 constexpr int FX_EPT = 8;
-constexpr int FX_TC = 16;
+constexpr int FX_TC = 8;
 
 // counter += add (one thread): advances device-side step counts between
 // kernels, so a captured graph can be replayed without new arguments
@@ -1387,30 +1393,46 @@ static __global__ void k_counter_add(long long *counter, int add) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter += add;
 }
 
+// tau[t * ne + e] for the block's nt steps, after n0 steps accumulated before it
+static __global__ __launch_bounds__(256) void k_flux_tau(double2 *__restrict__ tau,
+                                                  const double *__restrict__ energies, int ne, int nt,
+                                                  const long long *__restrict__ n0p, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt * ne) return;
+  const int t = i / ne, e = i % ne;
+  const double time = (double)(*n0p + t + 1) * dt;
+  double s, c;
+  sincos(energies[e] * time, &s, &c);
+  tau[i] = make_double2(c * dt, s * dt);
+}
+
 static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restrict__ bpsi,
                                                          const double2 *__restrict__ bdpsi,
                                                          double2 *__restrict__ fai, double2 *__restrict__ dfai,
-                                                         const double *__restrict__ energies, int nx, int ne,
-                                                         int nt, const long long *__restrict__ n0p, double dt) {
-  const long long n0 = *n0p;  // steps accumulated before this block (device counter)
+                                                         const double2 *__restrict__ tau, int nx, int ne, int nt) {
   __shared__ double2 sb[FX_TC][FX_X];
   __shared__ double2 sdb[FX_TC][FX_X];
   __shared__ double2 stau[FX_TC][FX_E];
+  // linear id = (xb / 8) * 8 * ney + eb * 8 + xb % 8
+  const int ney = (ne + FX_E - 1) / FX_E, lid = blockIdx.x;
+  const int xb = (lid / (8 * ney)) * 8 + lid % 8, eb = (lid / 8) % ney;
+  if (xb * FX_X >= nx) return;  // padding of the last group of eight x ranges (whole work group)
   const int t = threadIdx.x;
   const int xl = t & 63;
   const int eg = t >> 6;
-  const int x = blockIdx.x * FX_X + xl;
-  const int e0 = blockIdx.y * FX_E;
-  double2 af[FX_EPT], adf[FX_EPT];
+  const int e0 = eb * FX_E;
+  double2 af[2][FX_EPT], adf[2][FX_EPT];
 #pragma unroll
-  for (int q = 0; q < FX_EPT; q++) {
-    af[q] = make_double2(0.0, 0.0);
-    adf[q] = make_double2(0.0, 0.0);
-  }
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int q = 0; q < FX_EPT; q++) {
+      af[h][q] = make_double2(0.0, 0.0);
+      adf[h][q] = make_double2(0.0, 0.0);
+    }
   for (int tc = 0; tc < nt; tc += FX_TC) {
     for (int u = t; u < FX_TC * FX_X; u += 256) {
       const int tt = u / FX_X, xx = u % FX_X;
-      const int gx = blockIdx.x * FX_X + xx;
+      const int gx = xb * FX_X + xx;
       double2 b = make_double2(0.0, 0.0), db = b;
       if (tc + tt < nt && gx < nx) {
         b = bpsi[(size_t)(tc + tt) * nx + gx];
@@ -1421,36 +1443,41 @@ static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *_
     }
     for (int u = t; u < FX_TC * FX_E; u += 256) {
       const int tt = u / FX_E, ee = u % FX_E;
-      double2 tau = make_double2(0.0, 0.0);
-      if (tc + tt < nt && e0 + ee < ne) {
-        const double time = (double)(n0 + tc + tt + 1) * dt;
-        double s, c;
-        sincos(energies[e0 + ee] * time, &s, &c);
-        tau = make_double2(c * dt, s * dt);
-      }
-      stau[tt][ee] = tau;
+      stau[tt][ee] = (tc + tt < nt && e0 + ee < ne) ? tau[(size_t)(tc + tt) * ne + e0 + ee] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     const int tmax = min(FX_TC, nt - tc);
     for (int tt = 0; tt < tmax; tt++) {
-      const double2 b = sb[tt][xl], db = sdb[tt][xl];
+      const double2 b0 = sb[tt][xl], db0 = sdb[tt][xl], b1 = sb[tt][xl + 64], db1 = sdb[tt][xl + 64];
 #pragma unroll
       for (int q = 0; q < FX_EPT; q++) {
-        const double2 tau = stau[tt][eg * FX_EPT + q];
-        af[q] = cadd(af[q], cmul(tau, b));
-        adf[q] = cadd(adf[q], cmul(tau, db));
+        const double2 tw = stau[tt][eg * FX_EPT + q];
+        auto mac = [&](double2 &acc, const double2 v) {  // acc += tw v
+          acc.x = fma(tw.x, v.x, acc.x);
+          acc.x = fma(-tw.y, v.y, acc.x);
+          acc.y = fma(tw.x, v.y, acc.y);
+          acc.y = fma(tw.y, v.x, acc.y);
+        };
+        mac(af[0][q], b0);
+        mac(adf[0][q], db0);
+        mac(af[1][q], b1);
+        mac(adf[1][q], db1);
       }
     }
     __syncthreads();
   }
-  if (x < nx) {
 #pragma unroll
-    for (int q = 0; q < FX_EPT; q++) {
-      const int e = e0 + eg * FX_EPT + q;
-      if (e < ne) {
-        const size_t o = (size_t)e * nx + x;
-        fai[o] = cadd(fai[o], af[q]);
-        dfai[o] = cadd(dfai[o], adf[q]);
+  for (int h = 0; h < 2; h++) {
+    const int x = xb * FX_X + xl + 64 * h;
+    if (x < nx) {
+#pragma unroll
+      for (int q = 0; q < FX_EPT; q++) {
+        const int e = e0 + eg * FX_EPT + q;
+        if (e < ne) {
+          const size_t o = (size_t)e * nx + x;
+          fai[o] = cadd(fai[o], af[h][q]);
+          dfai[o] = cadd(dfai[o], adf[h][q]);
+        }
       }
     }
   }
