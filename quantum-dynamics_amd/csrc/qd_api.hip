@@ -1777,11 +1777,13 @@ int autotune_handle(qd_handle *h) {
     }
     h->tmx_tpw = pick;
   }
-  // fused theta kernel: the same choice of tiles per work group
+  // fused theta kernel: tile-blocks per work item (items of blocks sharing V
+  // are at least one tile per wave, so smaller values matter too)
   if (h->fuse_theta) {
     const long long ntiles = (h->n12 + 15) / 16;
     const int base = h->tfu_tpw;
-    std::vector<int> cands = {base, std::max(1, base / 2), base * 3 / 2, base * 2, base * 4};
+    std::vector<int> cands = {base,     std::max(1, base / 2), std::max(1, base / 4), std::max(1, base / 8),
+                              base * 3 / 2, base * 2,          base * 4};
     float best = 1e30f;
     int pick = base;
     for (int c : cands) {
