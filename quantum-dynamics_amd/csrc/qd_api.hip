@@ -898,6 +898,7 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
     }
   }
   // both classes fit: upload and size the launches
+  upad.resize(upad.size() + 64, 0.f);  // the 16-column parts of k_rpass_mx read up to 12 floats past a row
   QD_RC(upload(&h->mx_umat, upad.data(), upad.size()));
   for (int c = 0; c < 2; c++) {
     const HostSet &H = hs[c];

@@ -574,6 +574,32 @@ __device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int
 #undef QD_MXA
 }
 
+// Outputs o0 .. o0 + 15 of a unit (row stride gs4 = its padded order): for
+// units of more than 32 lines, whose bundle holds that one unit, three threads
+// share each point and each keeps 16 accumulators.  The matrix rows are read
+// 16 columns at a time, past the row end for the last part (into the next row
+// or the zero padding after the last matrix; those outputs are >= gs4 and not
+// kept).
+template <int SP>
+__device__ __forceinline__ void mx_part16(const float2 *stg, int line0, int g, int gs4, int o0, int p, kfloat *M,
+                                          float2 (&acc)[16]) {
+#pragma unroll
+  for (int n = 0; n < 16; n++) acc[n] = make_float2(0.f, 0.f);
+#pragma unroll 2
+  for (int k = 0; k < g; k++) {
+    const float2 x = stg[(line0 + k) * SP + p];
+#pragma unroll
+    for (int n4 = 0; n4 < 4; n4++) {
+      const qd_f4v w = *reinterpret_cast<kfloat4 *>(M + k * gs4 + o0 + 4 * n4);
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        acc[4 * n4 + c].x = fmaf(x.x, w[c], acc[4 * n4 + c].x);
+        acc[4 * n4 + c].y = fmaf(x.y, w[c], acc[4 * n4 + c].y);
+      }
+    }
+  }
+}
+
 // TWO: sized for two resident work groups per CU (at most 128 registers per
 // thread, LDS within half the CU's): the lines are loaded at the top of their
 // own iteration, and the other work group's bundle covers the latency.
@@ -651,7 +677,31 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
     fetch_desc(bn, bn < A.nbundle);
     __syncthreads();
     // K -> l: c_m = sum_K U[K][m] c_K, one (unit, point) per thread, in place
-    // (only this thread touches this point of the unit's lines)
+    // (only this thread touches this point of the unit's lines); units above
+    // 32 lines: three threads per point (16 outputs each), all reads of a
+    // point before any of its writes
+    if constexpr (GB > 32) {
+      static_assert(NT % (3 * N) == 0, "the three parts of a unit's points fill whole passes");
+      for (int i0 = 0; i0 < nu * 3 * N; i0 += NT) {
+        const int i = i0 + t;
+        const bool act = i < nu * 3 * N;  // uniform per wave
+        const int task = i / N, p = i % N, u = act ? task / 3 : 0, part = task % 3;
+        const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
+        const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
+        float2 res[16];
+        if (act && slot) {
+          kfloat *U = (kfloat *)(A.umat + (slot == 1 ? ms0 : ms1));
+          mx_part16<SP>(stg, line0, g, slot == 1 ? g40 : g41, 16 * part, p, U, res);
+        }
+        __syncthreads();
+        if (act && slot) {
+#pragma unroll
+          for (int n = 0; n < 16; n++)
+            if (16 * part + n < g) stg[(line0 + 16 * part + n) * SP + p] = res[n];
+        }
+        __syncthreads();
+      }
+    } else
     for (int i0 = 0; i0 < nu * N; i0 += NT) {
       const int i = i0 + t;
       if (i < nu * N) {  // uniform per wave (N is a multiple of 64)
@@ -700,6 +750,27 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
     }
     __syncthreads();
     // l -> K and store: c_K = sum_m U[K][m] c_m (rows of U^T), straight to HBM
+    if constexpr (GB > 32) {
+      for (int i0 = 0; i0 < nu * 3 * N; i0 += NT) {
+        const int i = i0 + t;
+        if (i < nu * 3 * N) {
+          const int task = i / N, p = i % N, u = task / 3, part = task % 3;
+          const int ud = __builtin_amdgcn_readfirstlane(s_unit[u]);
+          const int line0 = ud & 255, g = (ud >> 8) & 255, slot = ud >> 16;
+          if (slot) {
+            const int g4 = slot == 1 ? g40 : g41;
+            kfloat *UT = (kfloat *)(A.umat + (slot == 1 ? ms0 : ms1) + g4 * g4);
+            float2 res[16];
+            mx_part16<SP>(stg, line0, g, g4, 16 * part, p, UT, res);
+#pragma unroll
+            for (int n = 0; n < 16; n++)
+              if (16 * part + n < g) A.data[s_off[line0 + 16 * part + n] + p] = res[n];
+          } else if (part == 0) {
+            for (int K = 0; K < g; K++) A.data[s_off[line0 + K] + p] = stg[(line0 + K) * SP + p];
+          }
+        }
+      }
+    } else
     for (int i0 = 0; i0 < nu * N; i0 += NT) {
       const int i = i0 + t;
       if (i < nu * N) {
