@@ -4,10 +4,13 @@ Bars: FP64 psi and P(E) within 1e-8 relative of the oracle
 (|dP_e| <= 1e-8 * max(max_e |P|, 1e-2)); the mixed path within 1e-4 of the
 FP64 path on the same grid.
 """
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def rel_psi(a, b):
@@ -19,41 +22,64 @@ def crp_err(a, b):
 
 
 @pytest.fixture(scope="module")
-def cfg0(qd):
+def hh2_small(qd):
+    """H+H2 J=0 on 64x48x40, 2000 x 10 au."""
     return qd.problem.config(0)
 
 
 @pytest.mark.parametrize("pipeline", ["fused", "plain"])
 @pytest.mark.parametrize("variant", ["cpu", "cuda"])
-def test_fp64_short_matches_oracle(qd, oracle, cfg0, pipeline, variant):
+def test_fp64_short_matches_oracle(qd, oracle, hh2_small, pipeline, variant):
     steps = 30
-    print("case: cfg0 fp64", pipeline, variant, steps, flush=True)
-    psi_g, crp_g = qd.run(cfg0, steps=steps, precision="fp64", pipeline=pipeline, variant=variant)
-    psi_o, crp_o, _, _ = oracle.time_evolution(cfg0, total_steps=steps, variant=1 if variant == "cuda" else 0)
+    print("case: hh2_small fp64", pipeline, variant, steps, flush=True)
+    psi_g, crp_g = qd.run(hh2_small, steps=steps, precision="fp64", pipeline=pipeline, variant=variant)
+    psi_o, crp_o, _, _ = oracle.time_evolution(hh2_small, total_steps=steps, variant=1 if variant == "cuda" else 0)
     assert rel_psi(psi_g, psi_o) < 1e-10
     assert crp_err(crp_g, crp_o) < 1e-8
 
 
-def test_fp64_full_cfg0_matches_oracle(qd, oracle, cfg0):
-    print("case: cfg0 fp64 full 2000 steps", flush=True)
-    psi_g, crp_g = qd.run(cfg0, precision="fp64")
-    psi_o, crp_o, _, _ = oracle.time_evolution(cfg0)
+def test_fp64_full_cfg0_matches_oracle(qd, oracle, hh2_small):
+    print("case: hh2_small fp64 full 2000 steps", flush=True)
+    psi_g, crp_g = qd.run(hh2_small, precision="fp64")
+    psi_o, crp_o, _, _ = oracle.time_evolution(hh2_small)
     print("dpsi", rel_psi(psi_g, psi_o), "dP", crp_err(crp_g, crp_o))
     assert crp_err(crp_g, crp_o) < 1e-8
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_full_FH2_192x128x80_matches_cached_oracle(qd, prec):
+    """F+H2 J=0 on 192x128x80 for the whole 10000 x 5 au against the oracle's
+    result cached in tests/golden (about 40 min of the CPU oracle;
+    make_oracle_pe.py): FP64 P(E) within 1e-8 and the final |psi|^2 within
+    1e-10; mixed P(E) within 1e-4 of it."""
+    ref = np.load(os.path.join(GOLDEN, "oracle_pe_FH2_192x128x80.npz"))
+    p = qd.problem.config(1)
+    assert int(ref["steps"]) == p.total_steps
+    print("case: F+H2 192x128x80 full", prec, flush=True)
+    psi, crp = qd.run(p, precision=prec)
+    z = psi.view(np.complex128)
+    a2 = float(np.sum(np.abs(z) ** 2))
+    print("dP", np.max(np.abs(crp - ref["crp"])), "d|psi|^2", abs(a2 - float(ref["psi_abs2"])) / float(ref["psi_abs2"]))
+    assert float(np.max(ref["crp"])) > 0.5  # a non-trivial P(E)
+    if prec == "fp64":
+        assert crp_err(crp, ref["crp"]) < 1e-8
+        assert abs(a2 - float(ref["psi_abs2"])) < 1e-10 * float(ref["psi_abs2"])
+    else:
+        assert np.max(np.abs(crp - ref["crp"])) < 1e-4
+
+
 @pytest.mark.parametrize("split", [2, 3])
-def test_mixed_short_close_to_fp64(qd, cfg0, split):
-    print("case: cfg0 mixed split", split, flush=True)
-    psi_d, crp_d = qd.run(cfg0, steps=200, precision="fp64")
-    psi_m, crp_m = qd.run(cfg0, steps=200, precision="mixed", theta_split=split)
+def test_mixed_short_close_to_fp64(qd, hh2_small, split):
+    print("case: hh2_small mixed split", split, flush=True)
+    psi_d, crp_d = qd.run(hh2_small, steps=200, precision="fp64")
+    psi_m, crp_m = qd.run(hh2_small, steps=200, precision="mixed", theta_split=split)
     print("dpsi", rel_psi(psi_m, psi_d), "dP", np.max(np.abs(crp_m - crp_d)))
     assert rel_psi(psi_m, psi_d) < 1e-3
     assert np.max(np.abs(crp_m - crp_d)) < 1e-3
 
 
-def test_handle_resume_and_norm(qd, cfg0):
-    p = cfg0.copy()
+def test_handle_resume_and_norm(qd, hh2_small):
+    p = hh2_small.copy()
     p.dump1 = None
     p.dump2 = None
     with qd.Propagator(p, precision="fp64") as prop:
@@ -73,7 +99,7 @@ def test_handle_resume_and_norm(qd, cfg0):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("crp_every", [0, 20])
-def test_graph_replay_equals_eager(qd, cfg0, prec, crp_every):
+def test_graph_replay_equals_eager(qd, hh2_small, prec, crp_every):
     """Whole flux blocks replayed as one captured graph give the same psi and
     P(E) as the same kernels launched one by one.  Three propagate calls with
     host work between them replay the graph several times; with crp_every = 20
@@ -81,7 +107,7 @@ def test_graph_replay_equals_eager(qd, cfg0, prec, crp_every):
     print("case: graph replay", prec, "crp_every", crp_every, flush=True)
     out = []
     for graph in (0, 1):
-        with qd.Propagator(cfg0.copy(), precision=prec, flux_block=16, crp_every=crp_every,
+        with qd.Propagator(hh2_small.copy(), precision=prec, flux_block=16, crp_every=crp_every,
                            use_graph=graph) as prop:
             seen = []
             for n in (40, 33, 55):
@@ -96,23 +122,23 @@ def test_graph_replay_equals_eager(qd, cfg0, prec, crp_every):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_autotuned_handle_matches(qd, oracle, cfg0, prec):
+def test_autotuned_handle_matches(qd, oracle, hh2_small, prec):
     """The autotuner only picks among equivalent kernels: an autotuned FP64
     handle still matches the oracle, an autotuned mixed one the default."""
     print("case: autotune", prec, flush=True)
     steps = 40
-    with qd.Propagator(cfg0.copy(), precision=prec, autotune=1) as prop:
+    with qd.Propagator(hh2_small.copy(), precision=prec, autotune=1) as prop:
         tun = prop.tuning()
         prop.propagate(steps)
         psi_t, crp_t = prop.psi(), prop.crp()
     print("tuning", tun)
     assert tun["autotuned"] == 1
     if prec == "fp64":
-        psi_o, crp_o, _, _ = oracle.time_evolution(cfg0, total_steps=steps)
+        psi_o, crp_o, _, _ = oracle.time_evolution(hh2_small, total_steps=steps)
         assert rel_psi(psi_t, psi_o) < 1e-10
         assert crp_err(crp_t, crp_o) < 1e-8
     else:
-        psi_d, crp_d = qd.run(cfg0, steps=steps, precision="mixed")
+        psi_d, crp_d = qd.run(hh2_small, steps=steps, precision="mixed")
         assert rel_psi(psi_t, psi_d) < 1e-5
         assert np.max(np.abs(crp_t - crp_d)) < 1e-5
 
