@@ -38,7 +38,7 @@ def test_fp64_short_matches_oracle(qd, oracle, hh2_small, pipeline, variant):
     assert crp_err(crp_g, crp_o) < 1e-8
 
 
-def test_fp64_full_cfg0_matches_oracle(qd, oracle, hh2_small):
+def test_fp64_full_hh2_small_matches_oracle(qd, oracle, hh2_small):
     print("case: hh2_small fp64 full 2000 steps", flush=True)
     psi_g, crp_g = qd.run(hh2_small, precision="fp64")
     psi_o, crp_o, _, _ = oracle.time_evolution(hh2_small)
