@@ -520,13 +520,15 @@ typedef float qd_f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) float kfloat;
 typedef const __attribute__((address_space(4))) qd_f4v kfloat4;
 
-// acc_n = sum_k M[k][n] x_k over the G x G matrix M (row stride G, zero
-// padded beyond the unit's g lines) for the lines line0 .. line0 + g - 1 of
-// the staging rows at point p; out(n, acc_n) for n < g.  The k loop runs over
+// acc_n = sum_k M[k][n] x_k for the outputs n < G over the matrix M (row
+// stride GS = 4 ceil(G / 4), zero padded beyond the unit's g lines) for the
+// lines line0 .. line0 + g - 1 of the staging rows at point p; out(n, acc_n)
+// for n < g.  The k loop runs over
 // the g lines (g is uniform over the wave); its body has no branches, so the
 // compiler issues the loads ahead of the FMAs.
 template <int G, int SP, int NL, typename OUT>
 __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, int p, kfloat *M, OUT out) {
+  constexpr int GS = (G + 3) & ~3;  // row stride (padded order)
   float2 acc[G];
 #pragma unroll
   for (int n = 0; n < G; n++) acc[n] = make_float2(0.f, 0.f);
@@ -537,39 +539,42 @@ __device__ __forceinline__ void mx_apply(const float2 *stg, int line0, int g, in
   for (int k = 0; k < g; k++) {
     const float2 x = stg[(line0 + k) * SP + p];
 #pragma unroll
-    for (int n4 = 0; n4 < G / 4; n4++) {
-      const qd_f4v w = *reinterpret_cast<kfloat4 *>(M + k * G + 4 * n4);
-      acc[4 * n4].x = fmaf(x.x, w.x, acc[4 * n4].x);
-      acc[4 * n4].y = fmaf(x.y, w.x, acc[4 * n4].y);
-      acc[4 * n4 + 1].x = fmaf(x.x, w.y, acc[4 * n4 + 1].x);
-      acc[4 * n4 + 1].y = fmaf(x.y, w.y, acc[4 * n4 + 1].y);
-      acc[4 * n4 + 2].x = fmaf(x.x, w.z, acc[4 * n4 + 2].x);
-      acc[4 * n4 + 2].y = fmaf(x.y, w.z, acc[4 * n4 + 2].y);
-      acc[4 * n4 + 3].x = fmaf(x.x, w.w, acc[4 * n4 + 3].x);
-      acc[4 * n4 + 3].y = fmaf(x.y, w.w, acc[4 * n4 + 3].y);
+    for (int n4 = 0; n4 < GS / 4; n4++) {
+      const qd_f4v w = *reinterpret_cast<kfloat4 *>(M + k * GS + 4 * n4);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (4 * n4 + c < G) {
+          acc[4 * n4 + c].x = fmaf(x.x, w[c], acc[4 * n4 + c].x);
+          acc[4 * n4 + c].y = fmaf(x.y, w[c], acc[4 * n4 + c].y);
+        }
     }
   }
 #pragma unroll
   for (int n = 0; n < G; n++)
     if (n < g) out(n, acc[n]);
 }
-// mx_apply for the unit's padded order 4 ceil(g / 4) (at most GB)
+// mx_apply with the unit's g outputs rounded up to even, 2 ceil(g / 2) (at
+// most GB; rows of the matrix padded to a multiple of 4)
 template <int GB, int SP, int NL, typename OUT>
 __device__ __forceinline__ void mx_unit(const float2 *stg, int line0, int g, int p, kfloat *M, OUT out) {
 #define QD_MXA(G_) mx_apply<G_, SP, NL>(stg, line0, g, p, M, out)
-  switch ((g + 3) >> 2) {
-    case 1: QD_MXA(4); break;
-    case 2: if constexpr (GB >= 8) QD_MXA(8); break;
-    case 3: if constexpr (GB >= 12) QD_MXA(12); break;
-    case 4: if constexpr (GB >= 16) QD_MXA(16); break;
-    case 5: if constexpr (GB >= 20) QD_MXA(20); break;
-    case 6: if constexpr (GB >= 24) QD_MXA(24); break;
-    case 7: if constexpr (GB >= 28) QD_MXA(28); break;
-    case 8: if constexpr (GB >= 32) QD_MXA(32); break;
-    case 9: if constexpr (GB >= 36) QD_MXA(36); break;
-    case 10: if constexpr (GB >= 40) QD_MXA(40); break;
-    case 11: if constexpr (GB >= 44) QD_MXA(44); break;
-    default: if constexpr (GB >= 48) QD_MXA(48); break;
+  switch ((g + 1) >> 1) {
+    case 1: QD_MXA(2); break;
+    case 2: QD_MXA(4); break;
+    case 3: if constexpr (GB >= 8) QD_MXA(6); break;
+    case 4: if constexpr (GB >= 8) QD_MXA(8); break;
+    case 5: if constexpr (GB >= 12) QD_MXA(10); break;
+    case 6: if constexpr (GB >= 12) QD_MXA(12); break;
+    case 7: if constexpr (GB >= 16) QD_MXA(14); break;
+    case 8: if constexpr (GB >= 16) QD_MXA(16); break;
+    case 9: if constexpr (GB >= 20) QD_MXA(18); break;
+    case 10: if constexpr (GB >= 20) QD_MXA(20); break;
+    case 11: if constexpr (GB >= 24) QD_MXA(22); break;
+    case 12: if constexpr (GB >= 24) QD_MXA(24); break;
+    case 13: if constexpr (GB >= 28) QD_MXA(26); break;
+    case 14: if constexpr (GB >= 28) QD_MXA(28); break;
+    case 15: if constexpr (GB >= 32) QD_MXA(30); break;
+    default: if constexpr (GB >= 32) QD_MXA(32); break;
   }
 #undef QD_MXA
 }
