@@ -193,8 +193,8 @@ def roofline(kern, npt, workload, fused_theta, prec="mixed"):
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[workload][dom] * npt,
             "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt,
-            "note": ("the R pass with K<->l mixing is VALU-bound (about 159 VALU instructions per point: the "
-                     "register FFT plus the mixing in both directions; VALU issue about 67 % of its cycles), not "
+            "note": ("the R pass with K<->l mixing is VALU-bound (about 154 VALU instructions per point: the "
+                     "register FFT plus the mixing in both directions; VALU issue about 65 % of its cycles), not "
                      "HBM-bound; it moves its 16 algorithmic bytes per point" if (dom == "fft_R" and workload == "batched")
                      else "")}
 
