@@ -55,18 +55,28 @@ MEASURED_B_PER_POINT = {
     "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
     "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
 }
-TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER = 1, 2, 3, 4, 5
-KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER))
+TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER, TK_ACC = 1, 2, 3, 4, 5, 6
+KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER),
+            ("flux_accumulate", TK_ACC))
+FLUX_BLOCK = 64  # steps per time block of the flux accumulation (the library's default)
 
 
 def _run(prop, steps, warmup, profile):
+    """Times `steps` steps and everything they owe: the flux slices of the
+    timed steps are folded into phi(E), dphi(E) and P(E) is evaluated
+    (qd_get_crp) before the clock stops.  The warm-up's slices are folded
+    before the clock starts, so the timed region begins on an empty time
+    block: whole blocks inside it run as replayed hipGraphs, the rest eagerly."""
     if warmup:
         prop.propagate(warmup)
+    prop.crp()
     t0 = time.perf_counter()
     prop.propagate(steps)          # returns after the device has finished
+    prop.crp()                     # the last (partial) time block's accumulation and P(E)
     wall = time.perf_counter() - t0
     out = {"wall_ms_per_step": 1e3 * wall / steps, "event_ms_per_step": prop.timing(0),
-           "tuning": prop.tuning()}
+           "tuning": prop.tuning(), "flux_block": FLUX_BLOCK,
+           "flux_accumulations_timed": (steps + FLUX_BLOCK - 1) // FLUX_BLOCK}
     if profile:
         out["kernel_ms"] = {k: prop.timing(i) for k, i in KCLASSES}
     return out
@@ -76,13 +86,13 @@ def timed_single(qd, prob, precision, pipeline, steps, warmup, profile=0, autotu
     # whole flux blocks replayed as hipGraphs (the profiled runs launch eagerly,
     # with events between the kernels)
     with qd.Propagator(prob.copy(), precision=precision, pipeline=pipeline, profile=profile,
-                       autotune=autotune, use_graph=0 if profile else 1) as prop:
+                       autotune=autotune, use_graph=0 if profile else 1, flux_block=FLUX_BLOCK) as prop:
         return _run(prop, steps, warmup, profile)
 
 
 def timed_batched(qd, jk, precision, steps, warmup, profile=0, autotune=0):
     with qd.propagator.JKPropagator(jk, precision=precision, profile=profile, autotune=autotune,
-                                    use_graph=0 if profile else 1) as prop:
+                                    use_graph=0 if profile else 1, flux_block=FLUX_BLOCK) as prop:
         return _run(prop, steps, warmup, profile)
 
 
@@ -107,26 +117,77 @@ def plain_fp64_same_grids(qd, jk, steps, warmup):
     return ms, per_n, count
 
 
-def cpu_baseline(prob, seconds_target=15.0):
+def _host_cpus():
+    """Threads the oracle may use: the CPUs this process may run on, limited
+    by the cgroup CPU quota where one is set; with the CPU model."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return (min(n, quota) if quota else n), n, quota, model
+
+
+def _fftw_present():
+    import subprocess
+    try:
+        out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return False
+    return "libfftw3" in out
+
+
+def cpu_baseline(qd, block_sizes, seconds_target=20.0):
     """The oracle (C++/OpenMP restatement of timeEvol.C, reference-faithful:
-    P(E) every step and the per-step energy diagnostics on) on a bounded
-    number of steps of the same grid."""
+    P(E) and the energy diagnostics every step) timed on a sample of the
+    batched workload's blocks: one 256 x 192 x n grid for each of three theta
+    sizes n (the smallest, middle and largest block size), a few steps each;
+    the per-step time of every block is interpolated linearly in n from the
+    sample and summed over the blocks.  The reference propagates J = 0
+    only, so each block is timed as an uncoupled grid of its size (no K <-> l
+    mixing, which only favours the CPU)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as oracle
+    import numpy as np
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = min(cores, 16)
-    t0 = time.perf_counter()
-    oracle.time_evolution(prob, total_steps=2, crp_every=1, diagnostics=1, nthreads=cores)
-    t1 = (time.perf_counter() - t0) / 2
-    n = int(max(2, min(400, seconds_target / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    oracle.time_evolution(prob, total_steps=n, crp_every=1, diagnostics=1, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": prob.grid_points * n / dt, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
-            "sample": "%d steps of the %dx%dx%d grid (F+H2 J=0, P(E) and energies every step), C++/OpenMP "
-                      "oracle on %d threads, %.1f s; the batched workload's blocks have the same per-point cost"
-                      % (n, prob.n1, prob.n2, prob.ntheta, cores, dt)}
+    threads, nproc, quota, model = _host_cpus()
+    sizes = sorted(set(block_sizes))
+    sample = sorted({sizes[0], sizes[len(sizes) // 2], sizes[-1]})
+    per_n, desc, total_s = {}, [], 0.0
+    for n in sample:
+        prob = qd.problem.config(2, ntheta=n)
+        t0 = time.perf_counter()
+        oracle.time_evolution(prob, total_steps=1, crp_every=1, diagnostics=1, nthreads=threads)
+        t1 = time.perf_counter() - t0
+        k = int(max(2, min(200, (seconds_target / len(sample)) / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.time_evolution(prob, total_steps=k, crp_every=1, diagnostics=1, nthreads=threads)
+        dt = time.perf_counter() - t0
+        total_s += dt
+        per_n[n] = dt / k
+        desc.append("%d steps of 256x192x%d" % (k, n))
+    xs = np.array(sample, dtype=float)
+    ys = np.array([per_n[n] for n in sample])
+    step_s = float(sum(np.interp(n, xs, ys) for n in block_sizes))
+    points = 256 * 192 * sum(block_sizes)
+    return {"value": points / step_s, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
+            "s_per_step_extrapolated": step_s, "cpu_model": model, "nproc": nproc, "cgroup_cpu_quota": quota,
+            "fft": "FFTW present on the host but not used: the oracle's own mixed-radix FFT" if _fftw_present()
+                   else "the oracle's own mixed-radix FFT (no libfftw3 on the host)",
+            "sample": "C++/OpenMP oracle on %d threads (%s), P(E) and energies every step: %s (%.1f s); "
+                      "per-step time interpolated in n_theta and summed over the workload's %d block(s)"
+                      % (threads, model, ", ".join(desc), total_s, len(block_sizes))}
 
 
 def accuracy_report(qd, out_path=None):
@@ -330,7 +391,8 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob)
+        out["cpu_baseline"] = cpu_baseline(qd, [b.n for b in jk.blocks] if args.workload == "batched"
+                                           else [prob.ntheta])
     print(json.dumps(out))
 
 

@@ -152,9 +152,12 @@ int qd_get_psi(qd_handle *h, double *psi_out);          /* 2*n1*n2*ntheta double
 int qd_get_crp(qd_handle *h, double *crp_out);          /* n_energies doubles */
 int qd_get_norm(qd_handle *h, double *norm_out);        /* sum_k w_k sum |psi|^2 dr1 dr2 */
 int qd_get_steps(qd_handle *h, int32_t *steps_out);
-/* Average device time of one step over the last qd_propagate call (ms), and of
- * the dominant kernel class selected by `which` (0 = step, 1 = theta GEMMs,
- * 2 = R pass, 3 = r pass, 4 = flux). */
+/* Average device time of one step over the last qd_propagate call (ms), and,
+ * with options.profile = 1, of one launch of the kernel class selected by
+ * `which` (0 = step, 1 = theta GEMMs, 2 = R pass, 3 = r pass, 4 = flux-surface
+ * extraction, 5 = other, 6 = one flux accumulation, i.e. the fold of the
+ * time block into phi(E), dphi(E); flushes made by qd_get_crp after the call
+ * are included). */
 int qd_get_timing(qd_handle *h, int32_t which, double *ms_out);
 int qd_destroy(qd_handle *h);
 

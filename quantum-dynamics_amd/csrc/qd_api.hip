@@ -57,7 +57,7 @@ int fail(int code, const std::string &msg) {
     if (rc_ != QD_OK) return rc_; \
   } while (0)
 
-enum { TK_STEP = 0, TK_THETA = 1, TK_FFTR1 = 2, TK_FFTR2 = 3, TK_FLUX = 4, TK_OTHER = 5, TK_N = 6 };
+enum { TK_STEP = 0, TK_THETA = 1, TK_FFTR1 = 2, TK_FFTR2 = 3, TK_FLUX = 4, TK_OTHER = 5, TK_ACC = 6, TK_N = 7 };
 
 // bf16 round-to-nearest-even of a float, as the device conversion does
 uint16_t bf16_bits(float f) {
@@ -551,7 +551,7 @@ int materialize_psi(qd_handle *h) {
 
 int flush_flux(qd_handle *h) {
   if (!h->do_crp || h->slot == 0) return QD_OK;
-  QD_RC(mark(h, TK_FLUX));
+  QD_RC(mark(h, TK_ACC));
   hipLaunchKernelGGL(k_flux_tau, grid1((size_t)h->slot * h->ne), dim3(256), 0, h->stream, h->ftau, h->energies, h->ne,
                      h->slot, h->d_acc_steps, h->dt);
   const long long nxb = (h->nx + FX_X - 1) / FX_X, ney = (h->ne + FX_E - 1) / FX_E;
@@ -560,6 +560,7 @@ int flush_flux(qd_handle *h) {
                      (int)h->nx, h->ne, h->slot);
   hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, h->stream, h->d_acc_steps, h->slot);
   QD_HIP(hipGetLastError());
+  QD_RC(mark(h, -1));  // closes the accumulation's interval
   h->acc_steps += h->slot;
   h->slot = 0;
   return QD_OK;
@@ -567,6 +568,7 @@ int flush_flux(qd_handle *h) {
 
 int compute_crp(qd_handle *h) {
   QD_RC(flush_flux(h));
+  if (h->profile) QD_RC(drain_marks(h));
   const double sign = h->variant == QD_VARIANT_CUDA ? -1.0 : 1.0;
   hipLaunchKernelGGL(k_crp, dim3(h->ne, h->nb), dim3(256), 0, h->stream, h->fai, h->dfai, h->dcol_w, h->eta_sq,
                      h->d_blocks, h->crp, h->n1, (int)h->nx, h->ne, h->dr1 / h->mu2, sign);
@@ -1589,6 +1591,16 @@ int spec_from_jk(const qd_jk_problem *q, BatchSpec &S) {
   return QD_OK;
 }
 
+// Drops the captured flux block: every change of a launch choice after the
+// capture (qd_debug_select) must call it, or replays would keep running the
+// kernels captured before the change.
+void drop_graph(qd_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
 void release(qd_handle *h) {
   if (!h) return;
   void *bufs[] = {h->psi64,    h->lpsi64, h->psi32,     h->lpsi32,   h->V,      h->expV32,   h->dcol_vidx,
@@ -1599,8 +1611,7 @@ void release(qd_handle *h) {
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
                   h->mx_umat,  h->d_tfu_items};
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
+  drop_graph(h);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : h->ev)
@@ -1638,6 +1649,7 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
   QD_HIP(hipSetDevice(h->dev));
   if (nsteps < 0) return fail(QD_ERR_ARG, "n_steps < 0");
   if (nsteps == 0) return QD_OK;
+  const int32_t steps0 = h->steps;
   for (int k = 0; k < TK_N; k++) {
     h->t_ms[k] = 0;
     h->t_n[k] = 0;
@@ -1697,7 +1709,9 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
   QD_HIP(hipEventSynchronize(e1));
   float ms = 0;
   QD_HIP(hipEventElapsedTime(&ms, e0, e1));
-  h->last_step_ms = ms / nsteps;
+  // a callback may stop the loop early: average over the steps actually taken
+  const int32_t taken = h->steps - steps0;
+  h->last_step_ms = taken > 0 ? ms / taken : 0.0;
   return QD_OK;
 }
 
@@ -2055,6 +2069,7 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
       return fail(QD_ERR_ARG, "this handle has no prefetching mixing R pass");
     if (value == 1 && h->mxs[0].slots2 == 0) return fail(QD_ERR_ARG, "two work groups per CU do not fit this handle");
     if (value != 0 && value != 1) return fail(QD_ERR_ARG, "value must be 0 or 1");
+    drop_graph(h);
     h->rp_mx = true;
     h->rp_rf = false;
     h->mx_two = value == 1;
@@ -2062,6 +2077,7 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
   }
   if (what == 1) {
     if (value < 1 || value > 64) return fail(QD_ERR_ARG, "chunk must be 1..64");
+    drop_graph(h);
     h->mx_chunk = value;
     return QD_OK;
   }
