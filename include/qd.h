@@ -241,6 +241,15 @@ int qd_mlp_loss_grad(qd_mlp *h, float *loss, float *grad);
  * replays one captured iteration as a hipGraph; last_loss may be NULL */
 int qd_mlp_train(qd_mlp *h, int32_t iters, float lr, int32_t use_graph, float *last_loss);
 int qd_mlp_predict(qd_mlp *h, const float *X, int32_t n, float *Y);
+/* qd_mlp_predict that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate device memory, e.g. a tight serving loop that
+ * predicts P^J(E) of skipped J.  X and Y are host arrays as above; they are
+ * staged through a fixed device array of QD_MLP_NOALLOC_MAX samples (1.25 MiB,
+ * one per GPU, held until the process ends) instead of per-call buffers.
+ * n > QD_MLP_NOALLOC_MAX returns QD_ERR_ARG.  Calls that overlap on one GPU
+ * share that array and race. */
+#define QD_MLP_NOALLOC_MAX 65536
+int qd_mlp_predict_noalloc(qd_mlp *h, const float *X, int32_t n, float *Y);
 int qd_mlp_get_params(qd_mlp *h, float *params);
 int qd_mlp_destroy(qd_mlp *h);
 const char *qd_mlp_last_error(void);

@@ -15,7 +15,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -245,11 +247,49 @@ struct qd_mlp {
   float lr = 1e-3f;
 };
 
+// ---- staging for qd_mlp_predict_noalloc --------------------------------------
+// Inputs X[n][n_in] and outputs Y[n] of one call, at most QD_MLP_NOALLOC_MAX
+// samples: 5 floats per sample, 1.25 MiB.  Each GPU the library runs on holds
+// its own copy until the process ends.  In host code `scratch` names a host
+// stand-in, not device memory: only the address that hipGetSymbolAddress
+// returns for the current device may be passed to kernels and copies.
+constexpr int NOALLOC_FLOATS = QD_MLP_NOALLOC_MAX * 5;
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
 namespace {
-thread_local std::string g_mlp_err;
+// Error text in a fixed buffer, so that reporting an error allocates nothing.
+thread_local char g_mlp_err[512];
+void set_err(const char *a, const char *b = nullptr) {
+  snprintf(g_mlp_err, sizeof(g_mlp_err), "%s%s%s", a, b ? ": " : "", b ? b : "");
+}
 int mfail(int code, const std::string &msg) {
-  g_mlp_err = msg;
+  set_err(msg.c_str());
   return code;
+}
+int mfail_c(int code, const char *msg, const char *detail = nullptr) {
+  set_err(msg, detail);
+  return code;
+}
+
+// Device address of `scratch` per device ordinal, looked up once per device
+// (with that device current) by qd_mlp_create or the first no-allocation call.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_for_device(int dev, float **out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return mfail_c(QD_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "hipSetDevice", hipGetErrorString(e));
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return mfail_c(QD_ERR_DEVICE, "hipGetSymbolAddress(scratch)", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return QD_OK;
 }
 #define QM_HIP(x)                                                                     \
   do {                                                                                \
@@ -305,7 +345,7 @@ int launch_iteration(qd_mlp *h) {
 
 extern "C" {
 
-const char *qd_mlp_last_error(void) { return g_mlp_err.c_str(); }
+const char *qd_mlp_last_error(void) { return g_mlp_err; }
 
 int qd_mlp_n_params(int32_t n_in, int64_t *n_out) {
   if (!n_out) return mfail(QD_ERR_NULL, "null output");
@@ -325,6 +365,8 @@ int qd_mlp_create(int32_t n_in, const float *params, int32_t device, qd_mlp **ou
     return rc;
   };
   if (hipSetDevice(device) != hipSuccess) return bad(mfail(QD_ERR_DEVICE, "hipSetDevice failed"));
+  float *staging = nullptr;  // resolves this device's no-allocation staging address once
+  if (const int rc = scratch_for_device(device, &staging)) return bad(rc);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bad(mfail(QD_ERR_DEVICE, "stream"));
   const size_t nb = sizeof(float) * h->L.n;
@@ -438,6 +480,34 @@ int qd_mlp_predict(qd_mlp *h, const float *X, int32_t n, float *Y) {
   QM_HIP(hipStreamSynchronize(h->stream));
   QM_HIP(hipFree(dx));
   QM_HIP(hipFree(dy));
+  return QD_OK;
+}
+
+// qd_mlp_predict without allocation: the inputs and outputs are staged in the
+// device's `scratch` array instead of per-call device buffers.  Calls that
+// overlap on one GPU share that array and race.
+int qd_mlp_predict_noalloc(qd_mlp *h, const float *X, int32_t n, float *Y) {
+  if (!h || !X || !Y) return mfail_c(QD_ERR_NULL, "null argument");
+  if (n < 0 || n > QD_MLP_NOALLOC_MAX)
+    return mfail_c(QD_ERR_ARG, "qd_mlp_predict_noalloc: n must be in [0, QD_MLP_NOALLOC_MAX = 65536]");
+  if (n == 0) return QD_OK;
+  float *ptr = nullptr;
+  if (const int rc = scratch_for_device(h->dev, &ptr)) return rc;
+  hipError_t e = hipSetDevice(h->dev);
+  if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "hipSetDevice", hipGetErrorString(e));
+  const int nin = h->L.n_in;
+  float *dx = ptr;                                   // [n][n_in]
+  float *dy = ptr + (size_t)QD_MLP_NOALLOC_MAX * 4;  // [n]
+  e = hipMemcpyAsync(dx, X, sizeof(float) * (size_t)n * nin, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "staging X", hipGetErrorString(e));
+  hipLaunchKernelGGL(k_mlp_forward, dim3((n + TS - 1) / TS), dim3(THREADS), 0, h->stream, h->p, dx, n, nin, h->L,
+                     dy);
+  e = hipGetLastError();
+  if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "k_mlp_forward", hipGetErrorString(e));
+  e = hipMemcpyAsync(Y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->stream);
+  if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "reading Y", hipGetErrorString(e));
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return mfail_c(QD_ERR_DEVICE, "hipStreamSynchronize", hipGetErrorString(e));
   return QD_OK;
 }
 

@@ -37,6 +37,7 @@ def _lib():
         lib.qd_mlp_loss_grad.argtypes = [C.c_void_p, f32p, f32p]
         lib.qd_mlp_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_int32, f32p]
         lib.qd_mlp_predict.argtypes = [C.c_void_p, f32p, C.c_int32, f32p]
+        lib.qd_mlp_predict_noalloc.argtypes = [C.c_void_p, f32p, C.c_int32, f32p]
         lib.qd_mlp_get_params.argtypes = [C.c_void_p, f32p]
         lib.qd_mlp_destroy.argtypes = [C.c_void_p]
         lib.qd_mlp_last_error.restype = C.c_char_p
@@ -96,6 +97,14 @@ class MLP:
         x = _f32(X)
         y = np.empty(x.shape[0], np.float32)
         _check(self.lib, self.lib.qd_mlp_predict(self.h, _ptr(x), int(x.shape[0]), _ptr(y)))
+        return y
+
+    def predict_noalloc(self, X) -> np.ndarray:
+        """predict() through qd_mlp_predict_noalloc: no device allocation per
+        call, at most QD_MLP_NOALLOC_MAX (65536) samples."""
+        x = _f32(X)
+        y = np.empty(x.shape[0], np.float32)
+        _check(self.lib, self.lib.qd_mlp_predict_noalloc(self.h, _ptr(x), int(x.shape[0]), _ptr(y)))
         return y
 
     def params(self) -> np.ndarray:
