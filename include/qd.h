@@ -126,6 +126,12 @@ typedef struct qd_options {
                              the dump, the flux rows, the V phase and the next
                              theta-forward GEMM (the state between steps is then the FBR
                              array; psi is rebuilt from it when read); -1 = never */
+  int32_t diagnostics_every; /* >0: every N-th step also evaluates the reference's
+                             per-step diagnostics -- module, e_kin, e_rot, e_pot
+                             (timeEvol.C:619-647,671) -- at the same points of the
+                             step, on the device (qd_get_diagnostics).  A single
+                             J = 0 grid only (qd_create); such steps run one kernel
+                             per operator.  0 = never */
 } qd_options;
 
 /* Per-step callback, the equivalent of options.wave_to_matlab
@@ -152,6 +158,13 @@ int qd_get_psi(qd_handle *h, double *psi_out);          /* 2*n1*n2*ntheta double
 int qd_get_crp(qd_handle *h, double *crp_out);          /* n_energies doubles */
 int qd_get_norm(qd_handle *h, double *norm_out);        /* sum_k w_k sum |psi|^2 dr1 dr2 */
 int qd_get_steps(qd_handle *h, int32_t *steps_out);
+/* Diagnostics of the last step that evaluated them (options.diagnostics_every):
+ * out[0] = the step count after it, out[1] = module (sum_k w_k sum |psi|^2
+ * dr1 dr2, timeEvol.C:220-243, after the dump), out[2] = e_kin (:390-436, in
+ * momentum space after the kinetic phase), out[3] = e_rot (:323-360, after the
+ * second rotational half phase), out[4] = e_pot (:256-283, after the backward
+ * theta transform, before the dump).  QD_ERR_ARG if none has been evaluated. */
+int qd_get_diagnostics(qd_handle *h, double *out);
 /* Average device time of one step over the last qd_propagate call (ms), and,
  * with options.profile = 1, of one launch of the kernel class selected by
  * `which` (0 = step, 1 = theta GEMMs, 2 = R pass, 3 = r pass, 4 = flux-surface
@@ -177,6 +190,10 @@ typedef struct qd_jk_block {
   int32_t J, K;           /* 0 <= K <= min(J, jmax); every J needs K = 0..min(J, jmax) */
   const double *pot;      /* V on the block's grid, n1*n2*n_K (blocks of one K share it) */
   const double *psi;      /* initial psi, 2*n1*n2*n_K interleaved */
+  const double *eta_sq;   /* |eta_J(E)|^2 normalising this block's flux, n_energies
+                             values (the packet's projection onto the incoming wave
+                             of partial wave J, qd_init_packet), or NULL for
+                             base.crp.eta_sq */
 } qd_jk_block;
 
 typedef struct qd_jk_problem {

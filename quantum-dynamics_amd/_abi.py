@@ -48,11 +48,12 @@ class qd_options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("pipeline", C.c_int32), ("variant", C.c_int32),
                 ("flux_block", C.c_int32), ("crp_every", C.c_int32), ("use_graph", C.c_int32),
                 ("device", C.c_int32), ("profile", C.c_int32),
-                ("theta_split", C.c_int32), ("autotune", C.c_int32), ("fuse_theta", C.c_int32)]
+                ("theta_split", C.c_int32), ("autotune", C.c_int32), ("fuse_theta", C.c_int32),
+                ("diagnostics_every", C.c_int32)]
 
 
 class qd_jk_block(C.Structure):
-    _fields_ = [("J", C.c_int32), ("K", C.c_int32), ("pot", _dp), ("psi", _dp)]
+    _fields_ = [("J", C.c_int32), ("K", C.c_int32), ("pot", _dp), ("psi", _dp), ("eta_sq", _dp)]
 
 
 class qd_jk_problem(C.Structure):

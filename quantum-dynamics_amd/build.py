@@ -1,7 +1,7 @@
 """Compile libqd_hip.so for gfx950 in-tree (hipcc; no JIT cache).
 
 The translation units compile in parallel: the C-ABI host code, the MLP
-kernels, and qd_dispatch.hip once per kernel family (-DQD_PART=0..8).
+kernels, and qd_dispatch.hip once per kernel family (-DQD_PART=0..10).
 """
 from __future__ import annotations
 
@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libqd_hip.so")
-DISPATCH_PARTS = 9
+DISPATCH_PARTS = 11
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

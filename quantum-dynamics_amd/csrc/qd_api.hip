@@ -32,6 +32,7 @@
 #include "qd_kernels.hpp"
 #include "qd_dispatch.hpp"
 #include "qd_pack.hpp"
+#include "qd_records.hpp"
 #include "qd_tables.hpp"
 
 using namespace qd;
@@ -117,6 +118,7 @@ struct HostBlock {
   int mat = 0;                  // theta matrix set
   int vset = 0;                 // potential set (blocks of one K share it)
   const double *psi = nullptr;  // initial psi (interleaved), n12 * nd complex
+  const double *eta_sq = nullptr;  // |eta(E)|^2 normalising this block's flux (NULL: crp.eta_sq)
 };
 
 // Mixing unit group of the R pass: FBR columns (eigen slot m <-> column of K = m) and U.
@@ -234,6 +236,8 @@ struct qd_handle {
   // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
   // matrices and the V columns, and the work items built from them
   std::vector<std::pair<int, int>> tfu_groups;  // first block, blocks
+  std::vector<TfuBlock> tfu_blocks;             // the blocks in K order, for the bounds checks
+  long long nV = 0;                             // values of V (all potential sets)
   int4 *d_tfu_items = nullptr;
   int tfu_nitems = 0, tfu_items_cap = 0;
 
@@ -242,6 +246,15 @@ struct qd_handle {
   FftPlanArg planR{}, planr{};
   int rl_lpb = 16, rl_blocks = 0;
   size_t rl_lds = 0;
+  // the reference's per-step diagnostics (options.diagnostics_every): device
+  // tables [psq1 n1][psq2 n2][I1 n1][I2 n2][1/(l+1/2) nfcol][l(l+1)/(l+1/2) nfcol]
+  int diag_every = 0;
+  bool diag_ok = false, diag_have = false;
+  double *diag_tab = nullptr;
+  double diag_last[5] = {};
+  // FP64 staging for host <-> device copies of FP32 complex arrays (mixed
+  // precision): STAGE_POINTS values, allocated with the handle
+  double2 *stage = nullptr;
   double2 *bpsi = nullptr, *bdpsi = nullptr, *fai = nullptr, *dfai = nullptr;
   double2 *ftau = nullptr;  // tau[t][e] of the current time block (k_flux_tau)
   double *energies = nullptr, *eta_sq = nullptr, *crp = nullptr, *partial = nullptr;
@@ -267,6 +280,45 @@ int upload(T **dst, const T *src, size_t n) {
 }
 
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+constexpr size_t STAGE_POINTS = size_t(1) << 22;  // 4 Mi complex values: 64 MiB of FP64 staging
+
+// n host complex values (interleaved FP64) into the handle's complex array dst
+// at element offset off, in the handle's precision.  The mixed path converts
+// through the staging buffer in chunks, so no full-size temporary is made.
+int put_complex(qd_handle *h, void *dst, size_t off, const double *src, size_t n) {
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMemcpy(static_cast<double2 *>(dst) + off, src, sizeof(double2) * n, hipMemcpyHostToDevice));
+    return QD_OK;
+  }
+  for (size_t c = 0; c < n; c += STAGE_POINTS) {
+    const size_t m = std::min(STAGE_POINTS, n - c);
+    QD_HIP(hipMemcpyAsync(h->stage, src + 2 * c, sizeof(double2) * m, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_f64_to_f32, grid1(m), dim3(256), 0, h->stream, h->stage, static_cast<float2 *>(dst) + off + c,
+                       m);
+    QD_HIP(hipGetLastError());
+  }
+  QD_HIP(hipStreamSynchronize(h->stream));
+  return QD_OK;
+}
+// n complex values of the handle's array src (from element offset off) to the
+// host as interleaved FP64.
+int get_complex(qd_handle *h, const void *src, size_t off, double *dst, size_t n) {
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMemcpyAsync(dst, static_cast<const double2 *>(src) + off, sizeof(double2) * n, hipMemcpyDeviceToHost,
+                          h->stream));
+  } else {
+    for (size_t c = 0; c < n; c += STAGE_POINTS) {
+      const size_t m = std::min(STAGE_POINTS, n - c);
+      hipLaunchKernelGGL(k_f32_to_f64, grid1(m), dim3(256), 0, h->stream, static_cast<const float2 *>(src) + off + c,
+                         h->stage, m);
+      QD_HIP(hipGetLastError());
+      QD_HIP(hipMemcpyAsync(dst + 2 * c, h->stage, sizeof(double2) * m, hipMemcpyDeviceToHost, h->stream));
+    }
+  }
+  QD_HIP(hipStreamSynchronize(h->stream));
+  return QD_OK;
+}
 
 // ---- profiling: a chain of events, one before every kernel, drained in batches ----
 int drain_marks(qd_handle *h) {
@@ -513,22 +565,21 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
 // tiles in ranges of max(8, tfu_tpw / blocks) tiles, so that every item holds
 // about tfu_tpw tile-blocks.  Set-up and autotuning only (a synchronous copy).
 int build_tfu_items(qd_handle *h) {
-  const long long W = TMX_THREADS / 64, ntiles = ((long long)h->n12 + 15) / 16;
-  std::vector<int4> items;
-  for (const auto &g : h->tfu_groups) {
-    const long long tc = std::min(ntiles, std::max(W, ((long long)h->tfu_tpw + g.second - 1) / g.second));
-    for (long long t = 0; t < ntiles; t += tc)
-      items.push_back(make_int4(g.first, g.second, (int)t, (int)std::min(ntiles, t + tc)));
-  }
+  const long long ntiles = ((long long)h->n12 + 15) / 16;
+  const std::vector<TfuItem> items = qd::build_tfu_items(h->tfu_groups, ntiles, h->tfu_tpw, TMX_THREADS / 64);
+  const std::string err = check_tfu_items(items, h->tfu_blocks, (long long)h->n12, h->ndcol, h->nfcol, h->nV);
+  if (!err.empty()) return fail(QD_ERR_ARG, err);
   if ((int)items.size() > h->tfu_items_cap) {
+    QD_HIP(hipStreamSynchronize(h->stream));  // no launch still reads the old items
     if (h->d_tfu_items) QD_HIP(hipFree(h->d_tfu_items));
     h->d_tfu_items = nullptr;
     h->tfu_items_cap = 0;
     QD_HIP(hipMalloc((void **)&h->d_tfu_items, items.size() * sizeof(int4)));
     h->tfu_items_cap = (int)items.size();
   }
+  static_assert(sizeof(TfuItem) == sizeof(int4), "work items are read as int4");
   QD_HIP(hipStreamSynchronize(h->stream));
-  QD_HIP(hipMemcpy(h->d_tfu_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  QD_HIP(hipMemcpy(h->d_tfu_items, items.data(), items.size() * sizeof(TfuItem), hipMemcpyHostToDevice));
   h->tfu_nitems = (int)items.size();
   return QD_OK;
 }
@@ -581,37 +632,130 @@ int compute_crp(qd_handle *h) {
 }
 
 int apply_prephase(qd_handle *h) {
-  double2 *p64 = h->psi64;
-  bool tmp = false;
-  if (h->prec == QD_PREC_MIXED) {
-    QD_HIP(hipMalloc((void **)&p64, sizeof(double2) * h->npt));
-    tmp = true;
-    hipLaunchKernelGGL(k_f32_to_f64, grid1(h->npt), dim3(256), 0, h->stream, h->psi32, p64, h->npt);
+  // FP64 arithmetic on the stored values of either precision, in place
+  const bool cuda = h->variant == QD_VARIANT_CUDA;
+  if (h->prec == QD_PREC_FP64) {
+    if (cuda)
+      hipLaunchKernelGGL(k_prephase_cuda<double2>, grid1(h->npt), dim3(256), 0, h->stream, h->psi64, h->V,
+                         h->dcol_vidx, h->n12, h->dt, h->npt);
+    else
+      hipLaunchKernelGGL(k_prephase_cpu<double2>, grid1(h->npt), dim3(256), 0, h->stream, h->psi64, h->V,
+                         h->dcol_vidx, h->n12, h->dt, h->npt);
+  } else {
+    if (cuda)
+      hipLaunchKernelGGL(k_prephase_cuda<float2>, grid1(h->npt), dim3(256), 0, h->stream, h->psi32, h->V,
+                         h->dcol_vidx, h->n12, h->dt, h->npt);
+    else
+      hipLaunchKernelGGL(k_prephase_cpu<float2>, grid1(h->npt), dim3(256), 0, h->stream, h->psi32, h->V,
+                         h->dcol_vidx, h->n12, h->dt, h->npt);
   }
-  if (h->variant == QD_VARIANT_CUDA)
-    hipLaunchKernelGGL(k_prephase_cuda, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dcol_vidx, h->n12,
-                       h->dt, h->npt);
-  else
-    hipLaunchKernelGGL(k_prephase_cpu, grid1(h->npt), dim3(256), 0, h->stream, p64, h->V, h->dcol_vidx, h->n12,
-                       h->dt, h->npt);
   QD_HIP(hipGetLastError());
-  if (tmp) {
-    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, p64, h->psi32, h->npt);
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipFree(p64));
-  }
   return QD_OK;
 }
 
+enum { DG_MOD = 0, DG_KIN = 1, DG_ROT = 2, DG_POT = 3 };
+
+// One diagnostic partial sum (k_diag_sum) of a complex array of the handle's
+// precision into slot `slot` of h->partial.
 template <typename T>
-int step_impl(qd_handle *h) {
+int diag_sum(qd_handle *h, const typename cx_of<T>::type *a, size_t n, int slot) {
+  const int n1 = h->n1, n2 = h->n2;
+  const double *t = h->diag_tab;
+  const double *psq1 = t, *psq2 = t + n1, *I1 = t + n1 + n2, *I2 = t + 2 * n1 + n2;
+  const double *wkin = t + 2 * (n1 + n2), *wrot = wkin + h->nfcol;
+  const double *wc = slot == DG_KIN ? wkin : slot == DG_ROT ? wrot : h->dcol_w;
+  const double *fi = slot == DG_KIN ? psq1 : slot == DG_ROT ? I1 : nullptr;
+  const double *fj = slot == DG_KIN ? psq2 : slot == DG_ROT ? I2 : nullptr;
+  hipLaunchKernelGGL(k_diag_sum<T>, dim3(h->npartial), dim3(256), 0, h->stream, a, n, h->n12, n1, wc, fi, fj,
+                     slot == DG_POT ? h->V : nullptr, h->dcol_vidx, h->partial + (size_t)slot * h->npartial);
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+// The reference's operator sequence, one kernel per operator (evolution_dt,
+// timeEvol.C:619-647; dump_psi :713-735; the CUDA propagator dumps first,
+// evolutionCUDA2.cu:503).  With diag, the four diagnostics are reduced at the
+// points where the reference evaluates them.
+template <typename T>
+int plain_step(qd_handle *h, bool diag) {
   typedef typename cx_of<T>::type C;
   typedef typename tab_of<T>::type TB;
   C *psi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32);
   C *lpsi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32);
   const bool cuda = h->variant == QD_VARIANT_CUDA;
   const bool dump = h->has_dump;
-  if (h->fuse_theta) {
+  QD_RC(mark(h, TK_OTHER));
+  if (cuda && dump)
+    hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
+  hipLaunchKernelGGL(k_vphase<T>, grid1(h->npt), dim3(256), 0, h->stream, psi,
+                     h->prec == QD_PREC_FP64 ? h->V : nullptr, (const C *)h->expV32, h->dcol_vidx, h->n12, h->dt,
+                     h->npt);
+  QD_RC(mark(h, TK_THETA));
+  QD_RC(launch_theta(h, true, 0, 0));
+  QD_RC(mark(h, TK_OTHER));
+  hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
+                     h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
+  QD_RC(mark(h, TK_FFTR1));
+  QD_RC(launch_rpass<T>(h, FFT_FWD, lpsi));
+  QD_RC(mark(h, TK_FFTR2));
+  QD_RC(launch_rline<T>(h, FFT_FWD, lpsi));
+  QD_RC(mark(h, TK_OTHER));
+  hipLaunchKernelGGL(k_kin2d<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->kinR,
+                     (const TB *)h->kinr, h->n1, h->n2, h->nlp);
+  if (diag) QD_RC(diag_sum<T>(h, lpsi, h->nlp, DG_KIN));
+  QD_RC(mark(h, TK_FFTR2));
+  QD_RC(launch_rline<T>(h, FFT_INV, lpsi));
+  QD_RC(mark(h, TK_FFTR1));
+  QD_RC(launch_rpass<T>(h, FFT_INV, lpsi));
+  QD_RC(mark(h, TK_OTHER));
+  hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
+                     h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
+  if (diag) QD_RC(diag_sum<T>(h, lpsi, h->nlp, DG_ROT));
+  QD_RC(mark(h, TK_THETA));
+  QD_RC(launch_theta(h, false, 0, 0));
+  if (diag) QD_RC(diag_sum<T>(h, psi, h->npt, DG_POT));
+  if (!cuda && dump) {
+    QD_RC(mark(h, TK_OTHER));
+    hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
+  }
+  if (diag) QD_RC(diag_sum<T>(h, psi, h->npt, DG_MOD));
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+// Reads the four diagnostic sums of the step just enqueued (one host sync).
+int finish_diag(qd_handle *h, int32_t steps_after) {
+  std::vector<double> part(4 * (size_t)h->npartial);
+  QD_HIP(hipMemcpyAsync(part.data(), h->partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+  QD_HIP(hipStreamSynchronize(h->stream));
+  double sum[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; k++)
+    for (int b = 0; b < h->npartial; b++) sum[k] += part[(size_t)k * h->npartial + b];
+  const double da = h->dr1 * h->dr2;
+  h->diag_last[0] = steps_after;
+  h->diag_last[1] = sum[DG_MOD] * da;
+  h->diag_last[2] = sum[DG_KIN] * da * (double)h->n1 * h->n2;  // the kinetic table carries 1/(n1 n2)
+  h->diag_last[3] = sum[DG_ROT] * da;
+  h->diag_last[4] = sum[DG_POT] * da;
+  h->diag_have = true;
+  if (!std::isfinite(h->diag_last[1])) return fail(QD_ERR_NAN, "non-finite norm");
+  return QD_OK;
+}
+
+template <typename T>
+int step_impl(qd_handle *h, bool diag = false) {
+  typedef typename cx_of<T>::type C;
+  C *psi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32);
+  C *lpsi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32);
+  const bool cuda = h->variant == QD_VARIANT_CUDA;
+  const bool dump = h->has_dump;
+  if (diag) {
+    // the diagnostics need the intermediate arrays: this step runs one kernel
+    // per operator on psi; a fused-theta handle returns to its FBR state after it
+    QD_RC(materialize_psi(h));
+    h->state_fbr = false;
+    QD_RC(plain_step<T>(h, true));
+  } else if (h->fuse_theta) {
     // state on entry: the FBR array holds theta-forward(V psi_n); on exit the
     // same for psi_{n+1}.  The psi rows of the flux surface are written by the
     // fused kernel, after the dump, for k_flux_extract below.
@@ -621,6 +765,7 @@ int step_impl(qd_handle *h) {
     QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
     QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta_fused(h));
+    h->dvr_valid = false;
   } else if (h->pipe == QD_PIPE_FUSED) {
     // CUDA propagator: dump at the start of the step (evolutionCUDA2.cu:503),
     // fused with the potential phase; CPU propagator: dump after the step.
@@ -633,38 +778,7 @@ int step_impl(qd_handle *h) {
     QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta(h, false, 0, !cuda && dump ? 1 : 0));
   } else {
-    // one kernel per reference operator (timeEvol.C:619-647, dump_psi :713-735)
-    QD_RC(mark(h, TK_OTHER));
-    if (cuda && dump)
-      hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
-    hipLaunchKernelGGL(k_vphase<T>, grid1(h->npt), dim3(256), 0, h->stream, psi,
-                       h->prec == QD_PREC_FP64 ? h->V : nullptr, (const C *)h->expV32, h->dcol_vidx, h->n12, h->dt,
-                       h->npt);
-    QD_RC(mark(h, TK_THETA));
-    QD_RC(launch_theta(h, true, 0, 0));
-    QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
-                       h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
-    QD_RC(mark(h, TK_FFTR1));
-    QD_RC(launch_rpass<T>(h, FFT_FWD, lpsi));
-    QD_RC(mark(h, TK_FFTR2));
-    QD_RC(launch_rline<T>(h, FFT_FWD, lpsi));
-    QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_kin2d<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->kinR,
-                       (const TB *)h->kinr, h->n1, h->n2, h->nlp);
-    QD_RC(mark(h, TK_FFTR2));
-    QD_RC(launch_rline<T>(h, FFT_INV, lpsi));
-    QD_RC(mark(h, TK_FFTR1));
-    QD_RC(launch_rpass<T>(h, FFT_INV, lpsi));
-    QD_RC(mark(h, TK_OTHER));
-    hipLaunchKernelGGL(k_rot<T>, grid1(h->nlp), dim3(256), 0, h->stream, lpsi, (const TB *)h->dR, (const TB *)h->dr,
-                       h->fcol_lam, h->fcol_j, h->n1, h->n2, h->nlp);
-    QD_RC(mark(h, TK_THETA));
-    QD_RC(launch_theta(h, false, 0, 0));
-    if (!cuda && dump) {
-      QD_RC(mark(h, TK_OTHER));
-      hipLaunchKernelGGL(k_dump<T>, grid1(h->npt), dim3(256), 0, h->stream, psi, h->d1, h->d2, h->n1, h->n2, h->npt);
-    }
+    QD_RC(plain_step<T>(h, false));
   }
   QD_HIP(hipGetLastError());
   if (h->do_crp) {
@@ -674,6 +788,13 @@ int step_impl(qd_handle *h) {
     QD_HIP(hipGetLastError());
     h->slot++;
     if (h->slot == h->tblock) QD_RC(flush_flux(h));
+  }
+  if (diag) {
+    if (h->fuse_theta) {
+      QD_RC(enter_fbr_state(h));
+      h->dvr_valid = true;
+    }
+    QD_RC(finish_diag(h, h->steps + 1));  // labelled with the step count after this step
   }
   return QD_OK;
 }
@@ -713,6 +834,8 @@ int set_options(qd_handle *h, const qd_options *o) {
   h->nsplit = opt.theta_split == 2 ? 2 : 3;
   if (opt.fuse_theta != 0 && opt.fuse_theta != -1) return fail(QD_ERR_ARG, "fuse_theta must be 0 or -1");
   h->fuse_opt = opt.fuse_theta;
+  if (opt.diagnostics_every < 0) return fail(QD_ERR_ARG, "diagnostics_every < 0");
+  h->diag_every = opt.diagnostics_every;
   return QD_OK;
 }
 
@@ -818,99 +941,42 @@ int setup_theta_mx(qd_handle *h) {
 // matrices (shared by both classes) are stored padded to g4 = 4 ceil(g / 4),
 // U then U^T, rounded to FP32 with unit column lengths
 // (qdt::round_orthogonal_columns).
-int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam, int gmax) {
-  (void)gmax;
+int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam, int nlam) {
   const int n1 = h->n1, n2 = h->n2;
-  std::vector<float> upad;
-  std::map<int, std::pair<int, int>> mat_of;  // group u0 -> (offset, g4)
-  for (const Group &g : S.groups) {
-    const int gs = (int)g.cols.size();
-    if (g.u0 < 0 || gs < 2 || mat_of.count(g.u0)) continue;
-    const int g4 = (gs + 3) & ~3;
-    std::vector<float> q;
-    qdt::round_orthogonal_columns(S.umat.data() + g.u0, gs, q);
-    const int off = (int)upad.size();
-    upad.resize(upad.size() + 2 * (size_t)g4 * g4, 0.f);
-    for (int K = 0; K < gs; K++)
-      for (int m = 0; m < gs; m++) {
-        upad[off + (size_t)K * g4 + m] = q[(size_t)K * gs + m];                  // U[K][m]
-        upad[off + (size_t)g4 * g4 + (size_t)m * g4 + K] = q[(size_t)K * gs + m];  // U^T[m][K]
-      }
-    mat_of[g.u0] = {off, g4};
+  std::vector<std::vector<int>> gcols(S.groups.size());
+  std::vector<int> gu0(S.groups.size());
+  for (size_t i = 0; i < S.groups.size(); i++) {
+    gcols[i] = S.groups[i].cols;
+    gu0[i] = S.groups[i].u0;
   }
+  std::vector<float> upad;
+  MxRecords rec[2];
+  if (!build_mx_records(gcols, gu0, S.umat, fcol_lam, n1, n2, upad, rec)) return QD_OK;  // no mixing kernel
   int ncu = 0;
   QD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
   struct HostSet {
-    int nl = 0, gb = 0;
     size_t lds = 0;
     const void *fn = nullptr;
-    std::vector<MxHead> heads;
-    std::vector<long long> loff;
-    std::vector<int> llam, uni;
   } hs[2];
-  std::vector<int> sel[2];
-  for (size_t i = 0; i < S.groups.size(); i++) sel[S.groups[i].cols.size() > 32 ? 1 : 0].push_back((int)i);
   for (int c = 0; c < 2; c++) {
-    if (sel[c].empty()) continue;
-    HostSet &H = hs[c];
-    int gm = 1;
-    for (int gi : sel[c]) gm = std::max(gm, (int)S.groups[gi].cols.size());
-    H.nl = c == 0 ? 32 : 48;
-    H.gb = gm <= 8 ? 8 : gm <= 16 ? 16 : gm <= 24 ? 24 : gm <= 32 ? 32 : 48;
-    H.fn = rpass_mx_kernel(n1, H.nl, H.gb, false, &H.lds);
-    if (!H.fn || H.lds + (size_t)H.nl * 12 > 160 * 1024) return QD_OK;  // no mixing kernel for this batch
-    const int NL = H.nl;
-    MxHead cur{};
-    // units r0 .. r0 + count - 1 of group gi into the open bundle
-    auto add_units = [&](int gi, int r0, int count) {
-      const Group &g = S.groups[gi];
-      const int gs = (int)g.cols.size();
-      const bool mixed = g.u0 >= 0 && gs > 1;
-      int slot = 0;
-      if (mixed) {
-        const int src = mat_of[g.u0].first, g4 = mat_of[g.u0].second;
-        const int q = cur.mg4[0] ? 1 : 0;
-        cur.msrc[q] = src;
-        cur.mg4[q] = g4;
-        slot = q + 1;
-      }
-      const size_t bb = H.heads.size() * (size_t)NL;
-      for (int r = r0; r < r0 + count; r++) {
-        H.uni[bb + cur.nu] = cur.nl | gs << 8 | slot << 16;
-        for (int k = 0; k < gs; k++) {
-          const int col = g.cols[k];
-          H.loff[bb + cur.nl + k] = ((long long)col * n2 + r) * n1;
-          H.llam[bb + cur.nl + k] = fcol_lam[col];
-        }
-        cur.nl += gs;
-        cur.nu += 1;
-      }
-    };
-    std::vector<int> gsize(sel[c].size());
-    for (size_t i = 0; i < sel[c].size(); i++) gsize[i] = (int)S.groups[sel[c][i]].cols.size();
-    std::vector<std::vector<qdp::Run>> plan;
-    if (!qdp::pack_bundles(gsize, n2, NL, plan)) return QD_OK;
-    for (const auto &runs : plan) {
-      H.loff.resize(H.loff.size() + NL, 0);
-      H.llam.resize(H.llam.size() + NL, 0);
-      H.uni.resize(H.uni.size() + NL, 0);
-      for (const qdp::Run &run : runs) add_units(sel[c][run.group], run.r0, run.count);
-      if (cur.nl) H.heads.push_back(cur);
-      cur = MxHead{};
-    }
+    if (rec[c].heads.empty()) continue;
+    hs[c].fn = rpass_mx_kernel(n1, rec[c].nl, rec[c].gb, false, &hs[c].lds);
+    if (!hs[c].fn || hs[c].lds + (size_t)rec[c].nl * 12 > 160 * 1024) return QD_OK;  // no mixing kernel for this batch
+    const std::string err = check_mx_records(rec[c], h->nlp, n1, nlam, upad.size());
+    if (!err.empty()) return fail(QD_ERR_ARG, err);
   }
-  // both classes fit: upload and size the launches
-  upad.resize(upad.size() + 64, 0.f);  // the 16-column parts of k_rpass_mx read up to 12 floats past a row
+  // both classes fit and their records are in range: upload and size the launches
   QD_RC(upload(&h->mx_umat, upad.data(), upad.size()));
   for (int c = 0; c < 2; c++) {
     const HostSet &H = hs[c];
+    const MxRecords &R = rec[c];
     MxSet &M = h->mxs[c];
-    if (H.heads.empty()) continue;
-    QD_RC(upload(&M.head, H.heads.data(), H.heads.size()));
-    QD_RC(upload(&M.off, H.loff.data(), H.loff.size()));
-    QD_RC(upload(&M.lam, H.llam.data(), H.llam.size()));
-    QD_RC(upload(&M.unit, H.uni.data(), H.uni.size()));
-    const int nthreads = H.nl * rf_n1(n1);
+    if (R.heads.empty()) continue;
+    QD_RC(upload(&M.head, R.heads.data(), R.heads.size()));
+    QD_RC(upload(&M.off, R.loff.data(), R.loff.size()));
+    QD_RC(upload(&M.lam, R.llam.data(), R.llam.size()));
+    QD_RC(upload(&M.unit, R.uni.data(), R.uni.size()));
+    const int nthreads = R.nl * rf_n1(n1);
     int nb = 0;
     QD_HIP(hipFuncSetAttribute(H.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)H.lds));
     QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, H.fn, nthreads, H.lds));
@@ -919,17 +985,17 @@ int setup_mx(qd_handle *h, const BatchSpec &S, const std::vector<int> &fcol_lam,
     // the variant for two work groups per CU, where both fit
     M.slots2 = 0;
     size_t lds2 = 0;
-    const void *fn2 = rpass_mx_kernel(n1, H.nl, H.gb, true, &lds2);
+    const void *fn2 = rpass_mx_kernel(n1, R.nl, R.gb, true, &lds2);
     if (fn2 && lds2 == H.lds) {
       int nb2 = 0;
       QD_HIP(hipFuncSetAttribute(fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)H.lds));
       QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, fn2, nthreads, H.lds));
       if (nb2 >= 2) M.slots2 = (long long)nb2 * std::max(1, ncu);
     }
-    M.nl = H.nl;
-    M.gb = H.gb;
+    M.nl = R.nl;
+    M.gb = R.gb;
     M.lds = H.lds;
-    M.nbundle = (int)H.heads.size();
+    M.nbundle = (int)R.heads.size();
   }
   h->mx_two = h->mxs[0].slots2 > 0;  // measured faster where it fits; the autotuner re-checks
   h->rp_mx = true;
@@ -976,6 +1042,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     if (voff[b.vset] < 0) {
       voff[b.vset] = vtot;
       vtot += (long long)h->n12 * M.nd;
+      h->nV = vtot;
     }
     for (int k = 0; k < M.nd; k++) {
       dcol_vidx.push_back(voff[b.vset] + (long long)h->n12 * k);
@@ -1174,6 +1241,8 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_RC(upload(&h->d_blocks_k, byk.data(), byk.size()));
     // runs of blocks sharing the theta matrices, the basis sizes and the V columns
     h->tfu_groups.clear();
+    h->tfu_blocks.clear();
+    for (const BlockDesc &d : byk) h->tfu_blocks.push_back({d.nd, d.nf, d.dcol0, d.fcol0, d.mat, dcol_vidx[d.dcol0]});
     for (size_t i = 0; i < byk.size();) {
       size_t j = i + 1;
       while (j < byk.size() && byk[j].mat == byk[i].mat && byk[j].nd == byk[i].nd && byk[j].nf == byk[i].nf &&
@@ -1207,24 +1276,18 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   }
 
   // psi: blocks back to back
-  {
-    std::vector<double> host(2 * h->npt);
-    for (int b = 0; b < h->nb; b++) {
-      const BlockDesc &d = h->hblocks[b];
-      memcpy(host.data() + 2 * h->n12 * d.dcol0, S.blocks[b].psi, sizeof(double) * 2 * h->n12 * d.nd);
-    }
-    if (h->prec == QD_PREC_FP64) {
-      QD_RC(upload(&h->psi64, reinterpret_cast<const double2 *>(host.data()), h->npt));
-      QD_HIP(hipMalloc((void **)&h->lpsi64, sizeof(double2) * h->nlp));
-    } else {
-      double2 *tmp = nullptr;
-      QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(host.data()), h->npt));
-      QD_HIP(hipMalloc((void **)&h->psi32, sizeof(float2) * h->npt));
-      QD_HIP(hipMalloc((void **)&h->lpsi32, sizeof(float2) * h->nlp));
-      hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, tmp, h->psi32, h->npt);
-      QD_HIP(hipStreamSynchronize(h->stream));
-      QD_HIP(hipFree(tmp));
-    }
+  if (h->prec == QD_PREC_FP64) {
+    QD_HIP(hipMalloc((void **)&h->psi64, sizeof(double2) * h->npt));
+    QD_HIP(hipMalloc((void **)&h->lpsi64, sizeof(double2) * h->nlp));
+  } else {
+    QD_HIP(hipMalloc((void **)&h->stage, sizeof(double2) * std::min(STAGE_POINTS, std::max(h->npt, h->nlp))));
+    QD_HIP(hipMalloc((void **)&h->psi32, sizeof(float2) * h->npt));
+    QD_HIP(hipMalloc((void **)&h->lpsi32, sizeof(float2) * h->nlp));
+  }
+  for (int b = 0; b < h->nb; b++) {
+    const BlockDesc &d = h->hblocks[b];
+    QD_RC(put_complex(h, h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32, h->n12 * d.dcol0,
+                      S.blocks[b].psi, h->n12 * d.nd));
   }
   // potential sets (FP64; the mixed path adds an FP32 exp(-i V dt) table)
   {
@@ -1336,8 +1399,22 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   QD_RC(put_table(h, &h->kinr, kinr));
   QD_RC(put_table(h, &h->twR, twR));
   QD_RC(put_table(h, &h->twr, twr));
+  // diagnostics tables: a single J = 0 grid, FBR column c is Legendre degree l = c
+  h->diag_ok = S.blocks.size() == 1 && S.blocks[0].J == 0;
+  if (h->diag_every > 0 && !h->diag_ok)
+    return fail(QD_ERR_ARG, "diagnostics_every needs a single J = 0 grid (qd_create)");
+  if (h->diag_ok) {
+    std::vector<double> t;
+    for (int i = 0; i < n1; i++) t.push_back(p1v[i] * p1v[i] / (2 * p->r1.mass));
+    for (int j = 0; j < n2; j++) t.push_back(p2v[j] * p2v[j] / (2 * p->r2.mass));
+    t.insert(t.end(), I1.begin(), I1.end());
+    t.insert(t.end(), I2.begin(), I2.end());
+    for (int c = 0; c < fcol; c++) t.push_back(1.0 / (c + 0.5));
+    for (int c = 0; c < fcol; c++) t.push_back((double)c * (c + 1) / (c + 0.5));
+    QD_RC(upload(&h->diag_tab, t.data(), t.size()));
+  }
   QD_RC(upload(&h->fcol_lam, fcol_lam.data(), fcol_lam.size()));
-  if (mx_cand) QD_RC(setup_mx(h, S, fcol_lam, gmax));
+  if (mx_cand) QD_RC(setup_mx(h, S, fcol_lam, (int)lams.size()));
   if (h->rp_rf && h->has_mix) {
     // per-bundle line tables of the mixing R pass: element offset and D_R row per line
     std::vector<long long> loff;
@@ -1423,7 +1500,15 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
 
   if (h->do_crp) {
     QD_RC(upload(&h->energies, p->crp.energies, h->ne));
-    QD_RC(upload(&h->eta_sq, p->crp.eta_sq, h->ne));
+    {
+      // |eta(E)|^2 per block, [nb][ne]: a block's own array, else the problem's
+      std::vector<double> eta((size_t)h->nb * h->ne);
+      for (int b = 0; b < h->nb; b++) {
+        const double *src = S.blocks[b].eta_sq ? S.blocks[b].eta_sq : p->crp.eta_sq;
+        std::copy(src, src + h->ne, eta.begin() + (size_t)b * h->ne);
+      }
+      QD_RC(upload(&h->eta_sq, eta.data(), eta.size()));
+    }
     QD_HIP(hipMalloc((void **)&h->crp, sizeof(double) * h->ne * h->nb));
     QD_HIP(hipMalloc((void **)&h->d_acc_steps, sizeof(long long)));
     {
@@ -1438,7 +1523,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMemsetAsync(h->fai, 0, sizeof(double2) * h->nx * h->ne, h->stream));
     QD_HIP(hipMemsetAsync(h->dfai, 0, sizeof(double2) * h->nx * h->ne, h->stream));
   }
-  QD_HIP(hipMalloc((void **)&h->partial, sizeof(double) * h->npartial));
+  QD_HIP(hipMalloc((void **)&h->partial, sizeof(double) * 4 * h->npartial));  // 4 diagnostic slots
   QD_HIP(hipStreamSynchronize(h->stream));
   return QD_OK;
 }
@@ -1532,6 +1617,7 @@ int spec_from_jk(const qd_jk_problem *q, BatchSpec &S) {
     hb.mat = mat_of_key[key];
     hb.vset = vset_of_K[B.K];
     hb.psi = B.psi;
+    hb.eta_sq = B.eta_sq;
     S.blocks.push_back(hb);
   }
   // FBR columns: block (J, K), local index a <-> j = K + a
@@ -1610,7 +1696,7 @@ void release(qd_handle *h) {
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
-                  h->mx_umat,  h->d_tfu_items};
+                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab};
   drop_graph(h);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -1674,6 +1760,12 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
       const long long next = (h->steps / h->crp_every + 1) * h->crp_every;
       replay = next >= h->steps + B;
     }
+    // a diagnostics step runs eagerly (it reads its sums back at its end)
+    const bool diag = h->diag_every > 0 && (h->steps + 1) % h->diag_every == 0;
+    if (replay && h->diag_every > 0) {
+      const long long next = (h->steps / h->diag_every + 1) * h->diag_every;
+      replay = next > h->steps + B;
+    }
     if (replay) {
       if (!h->gexec) QD_RC(capture_block(h));
       QD_HIP(hipGraphLaunch(h->gexec, h->stream));
@@ -1681,8 +1773,8 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
       s += B;
       if (h->do_crp) h->acc_steps += B;
     } else {
-      if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h));
-      else QD_RC(step_impl<float>(h));
+      if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h, diag));
+      else QD_RC(step_impl<float>(h, diag));
       h->steps++;
       s++;
     }
@@ -1718,18 +1810,7 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
 int get_psi_impl(qd_handle *h, double *out) {
   QD_HIP(hipSetDevice(h->dev));
   QD_RC(materialize_psi(h));
-  if (h->prec == QD_PREC_FP64) {
-    QD_HIP(hipMemcpyAsync(out, h->psi64, sizeof(double2) * h->npt, hipMemcpyDeviceToHost, h->stream));
-  } else {
-    double2 *tmp = nullptr;
-    QD_HIP(hipMalloc((void **)&tmp, sizeof(double2) * h->npt));
-    hipLaunchKernelGGL(k_f32_to_f64, grid1(h->npt), dim3(256), 0, h->stream, h->psi32, tmp, h->npt);
-    QD_HIP(hipMemcpyAsync(out, tmp, sizeof(double2) * h->npt, hipMemcpyDeviceToHost, h->stream));
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipFree(tmp));
-  }
-  QD_HIP(hipStreamSynchronize(h->stream));
-  return QD_OK;
+  return get_complex(h, h->prec == QD_PREC_FP64 ? (const void *)h->psi64 : (const void *)h->psi32, 0, out, h->npt);
 }
 
 int norm_impl(qd_handle *h, double *out) {
@@ -1978,6 +2059,13 @@ int qd_get_steps(qd_handle *h, int32_t *steps_out) {
   return QD_OK;
 }
 
+int qd_get_diagnostics(qd_handle *h, double *out) {
+  if (!h || !out) return fail(QD_ERR_NULL, "NULL argument");
+  if (!h->diag_have) return fail(QD_ERR_ARG, "no step has evaluated the diagnostics (options.diagnostics_every)");
+  for (int k = 0; k < 5; k++) out[k] = h->diag_last[k];
+  return QD_OK;
+}
+
 int qd_get_timing(qd_handle *h, int32_t which, double *ms_out) {
   if (!h || !ms_out) return fail(QD_ERR_NULL, "NULL argument");
   if (which < 0 || which >= TK_N) return fail(QD_ERR_ARG, "bad timing selector");
@@ -2017,15 +2105,7 @@ extern "C" {
 int qd_set_psi(qd_handle *h, const double *psi_in) {
   if (!h || !psi_in) return fail(QD_ERR_NULL, "NULL argument");
   QD_HIP(hipSetDevice(h->dev));
-  if (h->prec == QD_PREC_FP64) {
-    QD_HIP(hipMemcpy(h->psi64, psi_in, sizeof(double2) * h->npt, hipMemcpyHostToDevice));
-  } else {
-    double2 *tmp = nullptr;
-    QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(psi_in), h->npt));
-    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->npt), dim3(256), 0, h->stream, tmp, h->psi32, h->npt);
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipFree(tmp));
-  }
+  QD_RC(put_complex(h, h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32, 0, psi_in, h->npt));
   h->state_fbr = false;
   h->dvr_valid = true;
   return QD_OK;
@@ -2034,32 +2114,13 @@ int qd_set_psi(qd_handle *h, const double *psi_in) {
 int qd_debug_get_fbr(qd_handle *h, double *out) {
   if (!h || !out) return fail(QD_ERR_NULL, "NULL argument");
   QD_HIP(hipSetDevice(h->dev));
-  if (h->prec == QD_PREC_FP64) {
-    QD_HIP(hipMemcpy(out, h->lpsi64, sizeof(double2) * h->nlp, hipMemcpyDeviceToHost));
-  } else {
-    double2 *tmp = nullptr;
-    QD_HIP(hipMalloc((void **)&tmp, sizeof(double2) * h->nlp));
-    hipLaunchKernelGGL(k_f32_to_f64, grid1(h->nlp), dim3(256), 0, h->stream, h->lpsi32, tmp, h->nlp);
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipMemcpy(out, tmp, sizeof(double2) * h->nlp, hipMemcpyDeviceToHost));
-    QD_HIP(hipFree(tmp));
-  }
-  return QD_OK;
+  return get_complex(h, h->prec == QD_PREC_FP64 ? (const void *)h->lpsi64 : (const void *)h->lpsi32, 0, out, h->nlp);
 }
 
 int qd_debug_set_fbr(qd_handle *h, const double *in) {
   if (!h || !in) return fail(QD_ERR_NULL, "NULL argument");
   QD_HIP(hipSetDevice(h->dev));
-  if (h->prec == QD_PREC_FP64) {
-    QD_HIP(hipMemcpy(h->lpsi64, in, sizeof(double2) * h->nlp, hipMemcpyHostToDevice));
-  } else {
-    double2 *tmp = nullptr;
-    QD_RC(upload(&tmp, reinterpret_cast<const double2 *>(in), h->nlp));
-    hipLaunchKernelGGL(k_f64_to_f32, grid1(h->nlp), dim3(256), 0, h->stream, tmp, h->lpsi32, h->nlp);
-    QD_HIP(hipStreamSynchronize(h->stream));
-    QD_HIP(hipFree(tmp));
-  }
-  return QD_OK;
+  return put_complex(h, h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32, 0, in, h->nlp);
 }
 
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {

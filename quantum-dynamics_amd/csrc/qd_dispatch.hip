@@ -8,8 +8,10 @@
 //   4  mixed theta GEMM
 //   5  mixing R pass with prefetch, 256-point lines
 //   6  mixing R pass with prefetch, 64- and 128-point lines
-//   7  register-FFT R pass with LDS-staged mixing, FP32
-//   8  register-FFT R pass with LDS-staged mixing, FP64
+//   7  register-FFT R pass with LDS-staged mixing, FP32, lengths up to 96
+//   8  register-FFT R pass with LDS-staged mixing, FP64, lengths up to 96
+//   9  register-FFT R pass with LDS-staged mixing, FP32, lengths 128..512
+//  10  register-FFT R pass with LDS-staged mixing, FP64, lengths 128..512
 // Every kernel instantiation lives in exactly one part.
 #include <hip/hip_runtime.h>
 
@@ -19,7 +21,7 @@
 #include "qd_rfft.hpp"
 
 #ifndef QD_PART
-#error "compile with -DQD_PART=<0..8>"
+#error "compile with -DQD_PART=<0..10>"
 #endif
 
 namespace qd {
@@ -134,8 +136,9 @@ template const void *rline_rf_kernel<double>(int, size_t *, int *);
 #endif
 #endif  // register-FFT parts
 
-#if QD_PART == 7 || QD_PART == 8
-// register-FFT R pass with K <-> l mixing staged in LDS (lw lines per work group)
+#if QD_PART >= 7 && QD_PART <= 10
+// register-FFT R pass with K <-> l mixing staged in LDS (lw lines per work group);
+// parts 7 / 8 (FP32 / FP64) hold lengths up to 96, parts 9 / 10 the longer ones
 template <typename T, int N1, int N2>
 const void *rpass_rf_mix_fn(int lw, size_t *lds) {
   constexpr int LW0 = 256 / N1;
@@ -149,16 +152,18 @@ const void *rpass_rf_mix_fn(int lw, size_t *lds) {
   }
   return nullptr;
 }
-template <typename T>
-const void *rpass_rf_mix_kernel(int n, int lw, size_t *lds) {
-  switch (n) {
+#define QD_RF_LENGTHS_LO(X) X(24, 8) X(32, 8) X(48, 8) X(64, 8) X(96, 16)
+#define QD_RF_LENGTHS_HI(X) X(128, 16) X(192, 16) X(256, 16) X(384, 32) X(512, 32)
 #define QD_CASE(N, N1) \
   case N:              \
     return rpass_rf_mix_fn<T, N1, N / N1>(lw, lds);
-    QD_RF_LENGTHS(QD_CASE)
-#undef QD_CASE
+#if QD_PART == 7 || QD_PART == 8
+template <typename T>
+const void *rpass_rf_mix_kernel(int n, int lw, size_t *lds) {
+  switch (n) {
+    QD_RF_LENGTHS_LO(QD_CASE)
     default:
-      return nullptr;
+      return rpass_rf_mix_kernel_long<T>(n, lw, lds);
   }
 }
 #if QD_PART == 7
@@ -166,6 +171,22 @@ template const void *rpass_rf_mix_kernel<float>(int, int, size_t *);
 #else
 template const void *rpass_rf_mix_kernel<double>(int, int, size_t *);
 #endif
+#else
+template <typename T>
+const void *rpass_rf_mix_kernel_long(int n, int lw, size_t *lds) {
+  switch (n) {
+    QD_RF_LENGTHS_HI(QD_CASE)
+    default:
+      return nullptr;
+  }
+}
+#if QD_PART == 9
+template const void *rpass_rf_mix_kernel_long<float>(int, int, size_t *);
+#else
+template const void *rpass_rf_mix_kernel_long<double>(int, int, size_t *);
+#endif
+#endif
+#undef QD_CASE
 #endif  // register-FFT mixing parts
 
 #if QD_PART == 4

@@ -22,6 +22,9 @@ const void *rpass_rf_kernel(int n, bool mix, int lw, size_t *lds);
 // the same with K <-> l mixing staged in LDS (lw lines per work group)
 template <typename T>
 const void *rpass_rf_mix_kernel(int n, int lw, size_t *lds);
+// its lengths above 96 (a separate build part)
+template <typename T>
+const void *rpass_rf_mix_kernel_long(int n, int lw, size_t *lds);
 // register-FFT r pass: *lds = dynamic LDS bytes, *lpb = lines per work group
 template <typename T>
 const void *rline_rf_kernel(int n, size_t *lds, int *lpb);

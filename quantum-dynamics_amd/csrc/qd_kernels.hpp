@@ -117,8 +117,9 @@ struct BlockDesc {
 // propagator (timeEvol.C:519-538).  sqrt_ref is the reference's pow(c, 0.5)
 // with arg() = asin(Im/|c|) plus quadrant fixes (complex.h:60-68,148-159); it
 // equals exp(+i V dt/2) only while |V dt| < pi, which is reproduced here.
-static __global__ void k_prephase_cpu(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
-                               double dt, size_t n) {
+template <typename C>
+__global__ void k_prephase_cpu(C *psi, const double *V, const long long *dcol_vidx, size_t n12, double dt,
+                               size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const double v = V[dcol_vidx[x / n12] + (long long)(x % n12)];
@@ -135,22 +136,27 @@ static __global__ void k_prephase_cpu(double2 *psi, const double *V, const long 
   const double2 q = make_double2(r * sq_c, r * sq_s);
   // c1 / abs2(c2) * conj(c2)
   const double q2 = q.x * q.x + q.y * q.y;
-  double2 p = psi[x];
+  double2 p = make_double2(psi[x].x, psi[x].y);  // FP64 arithmetic for either storage type
   p.x /= q2;
   p.y /= q2;
-  psi[x] = cmul(p, make_double2(q.x, -q.y));
+  p = cmul(p, make_double2(q.x, -q.y));
+  psi[x].x = p.x;
+  psi[x].y = p.y;
 }
 
 // psi *= exp(+i V dt/2): the CUDA propagator's first-step phase
 // (evolutionCUDA2.cu:507, evolution_with_potential(-dt/2)).
-static __global__ void k_prephase_cuda(double2 *psi, const double *V, const long long *dcol_vidx, size_t n12,
-                                double dt, size_t n) {
+template <typename C>
+__global__ void k_prephase_cuda(C *psi, const double *V, const long long *dcol_vidx, size_t n12, double dt,
+                                size_t n) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const double v = V[dcol_vidx[x / n12] + (long long)(x % n12)];
   double s, c;
   sincos(0.5 * dt * v, &s, &c);
-  psi[x] = cmul(psi[x], make_double2(c, s));
+  const double2 p = cmul(make_double2(psi[x].x, psi[x].y), make_double2(c, s));
+  psi[x].x = p.x;
+  psi[x].y = p.y;
 }
 
 // psi *= exp(-i V dt) (evolution_with_potential_dt, timeEvol.C:540-559); FP64
@@ -1505,7 +1511,7 @@ static __global__ __launch_bounds__(256) void k_crp(const double2 *__restrict__ 
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) crp[(size_t)blockIdx.y * ne + e] = sign * red[0] / eta_sq[e] * dr1_mu2;
+  if (threadIdx.x == 0) crp[(size_t)blockIdx.y * ne + e] = sign * red[0] / eta_sq[(size_t)blockIdx.y * ne + e] * dr1_mu2;
 }
 
 // per-group partial sums of w_k |psi|^2 (module_for_psi, timeEvol.C:220-243)
@@ -1518,6 +1524,40 @@ __global__ __launch_bounds__(256) void k_norm_partial(const typename cx_of<T>::t
   for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (size_t)gridDim.x * 256) {
     const typename cx_of<T>::type v = psi[x];
     s += dcol_w[x / n12] * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// One partial sum per work group of  wc[c] * f * |a[x]|^2  over the n points of
+// a complex array with columns of n12 = n1 * n2 points (c = x / n12, i = x mod
+// n1, j = (x / n1) mod n2), where f = fi[i] + fj[j] when fi is given, times
+// V[dcol_vidx[c] + x mod n12] when V is given.  This is the common form of the
+// reference's per-step diagnostics: module_for_psi (timeEvol.C:220-243, wc = w_k),
+// potential_energy (:256-283, wc = w_k, V), kinetic_energy_for_legendre_psi
+// (:390-436, wc = 1/(l+1/2), f = p1^2/2mu1 + p2^2/2mu2) and rotational_energy
+// (:323-360, wc = l(l+1)/(l+1/2), f = 1/2mu1R^2 + 1/2mu2r^2).
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_sum(const typename cx_of<T>::type *__restrict__ a, size_t n,
+                                                  size_t n12, int n1, const double *__restrict__ wc,
+                                                  const double *__restrict__ fi, const double *__restrict__ fj,
+                                                  const double *__restrict__ V,
+                                                  const long long *__restrict__ dcol_vidx,
+                                                  double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (size_t)gridDim.x * 256) {
+    const size_t c = x / n12, r = x - c * n12;
+    double f = wc[c];
+    if (fi) f *= fi[r % n1] + fj[r / n1];
+    if (V) f *= V[dcol_vidx[c] + (long long)r];
+    const typename cx_of<T>::type v = a[x];
+    s += f * ((double)v.x * v.x + (double)v.y * v.y);
   }
   red[threadIdx.x] = s;
   __syncthreads();

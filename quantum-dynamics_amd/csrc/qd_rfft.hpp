@@ -21,6 +21,7 @@
 #include <utility>
 
 #include "qd_kernels.hpp"
+#include "qd_records.hpp"
 
 namespace qd {
 
@@ -477,12 +478,8 @@ constexpr size_t rpass_rf_lds() {
 // work group per CU its lines are loaded during the FFT and the l -> K mixing,
 // so the HBM traffic of one bundle overlaps the arithmetic of the previous
 // one; with two work groups per CU (TWO) the other work group covers it.
-struct MxHead {
-  int nl, nu;    // lines and units of the bundle
-  int msrc[2];   // offsets in umat of its matrices (U, then U^T at + g4^2)
-  int mg4[2];    // their padded orders (multiples of 4; 0 = unused)
-  int pad[2];
-};
+// (MxHead, the bundle header, is in qd_records.hpp with the host code that
+// builds and checks the records.)
 
 // A work group of the mixing R pass takes `chunk` consecutive bundles at a
 // time (round-robin over the grid by chunks): consecutive bundles mostly share

@@ -216,9 +216,8 @@ class JInterpolator:
 
 def propagate_pj(qd, base, system, Js, jmax: int, steps: int, chunk: int = 1000, log=None, **jk_kw):
     """P^J(E) of every J in Js from one batched mixed-precision propagation
-    (all K blocks of every J, Coriolis-coupled; each J normalised by the
-    packet's |eta|^2 measured from its centrifugal energy), and the seconds it
-    took:
+    (all K blocks of every J, Coriolis-coupled; each J's flux normalised by
+    the packet's |eta_J(E)|^2 in partial wave J), and the seconds it took:
     problem set-up, handle creation (upload, tables, autotuning) and the
     propagation itself."""
     from . import problem as pbm
@@ -236,7 +235,7 @@ def propagate_pj(qd, base, system, Js, jmax: int, steps: int, chunk: int = 1000,
             done += n
             if log:
                 log("  J %s: %d / %d steps, %.1f s" % (_jrange(Js), done, steps, time.perf_counter() - t2))
-        PJ = prop.crp_by_J(eta="centrifugal")
+        PJ = prop.crp_by_J()
     t3 = time.perf_counter()
     points = int(jk.psi.size)
     del jk

@@ -10,6 +10,7 @@ propagator (``H3/main.m:24-140``).  Each helper restates one reference script:
 * ``diatomic_state``       H3/H2VibRotWaveFunction.m:4-47 (H3/H2PES.m:3-9 for the potential)
 * ``init_wave_packet``     H3/InitWavePacket.m:2-31
 * ``eta_sq``               common/EtaSq.m:4-18
+* ``eta_sq_partial_wave``  the same projection in partial wave J > 0 (reduces to EtaSq.m at J = 0)
 * ``jacobi_to_bonds``      H3/H3PESJacobi.m:6-29
 * ``leps``                 OHCl/OHClKSG.F:89-150 (Q/J form), constant r_e and beta
 * ``n_dividing_surface``   H3/main.m:77 (MATLAB int32() rounds half away from zero)
@@ -248,6 +249,54 @@ def eta_sq(delta: float, k0: float, mass: float, E: np.ndarray) -> np.ndarray:
     return np.abs(eta) ** 2
 
 
+def riccati_hankel_plus(l: int, x) -> np.ndarray:
+    """w_l(x) = x h_l^(1)(x) = x (j_l(x) + i y_l(x)), the outgoing Riccati-Hankel
+    function, by the upward recursion w_{l+1} = (2l+1)/x w_l - w_{l-1} from
+    w_0 = -i e^{ix}, w_1 = -(1 + i/x) e^{ix} (stable: h_l is dominated by the
+    irregular y_l).  |w_l(x)| -> 1 for x >> l."""
+    x = np.asarray(x, dtype=np.float64)
+    e = np.exp(1j * x)
+    w0 = -1j * e
+    if l == 0:
+        return w0
+    w1 = -(1.0 + 1j / x) * e
+    for n in range(1, l):
+        w0, w1 = w1, (2 * n + 1) / x * w1 - w0
+    return w1
+
+
+def eta_sq_partial_wave(delta: float, R0: float, k0: float, mass: float, E, J: int,
+                        n_quad: int = 4001, width: float = 10.0) -> np.ndarray:
+    """|eta_J(E)|^2 of the packet g(R) = (pi delta^2)^(-1/4) exp(-(R-R0)^2/2delta^2 - i k0 R)
+    (H3/InitWavePacket.m:13-17) in partial wave J:
+
+        |eta_J(E)|^2 = (mass / k) |int w_J(k R) g(R) dR|^2,   k = sqrt(2 mass E),
+
+    the projection onto the energy-normalised incoming free wave of orbital
+    angular momentum J (Riccati-Hankel w_J^- = conj(w_J)), in the normalisation
+    of common/EtaSq.m:8-16, to which it reduces exactly at J = 0 (w_0 = -i e^{ikR}).
+    For a packet far outside the centrifugal barrier it tends to the plane-wave
+    form evaluated at the local momentum k_J(R0) = sqrt(k^2 - J(J+1)/R0^2),
+    i.e. at E - J(J+1)/(2 mass R0^2) (the leading WKB term).  E <= 0 gives 0.
+    Quadrature: trapezoid over R0 +- width*delta (spectrally accurate for the
+    Gaussian-weighted integrand)."""
+    E = np.asarray(E, dtype=np.float64).reshape(-1)
+    out = np.zeros(E.size)
+    ok = E > 0
+    if not np.any(ok):
+        return out
+    R = np.linspace(R0 - width * delta, R0 + width * delta, n_quad)
+    R = R[R > 0]
+    dR = R[1] - R[0]
+    g = (1.0 / (np.pi * delta ** 2)) ** 0.25 * np.exp(-(R - R0) ** 2 / (2 * delta * delta) - 1j * k0 * R)
+    k = np.sqrt(2 * mass * E[ok])
+    w = riccati_hankel_plus(J, k[:, None] * R[None, :])
+    f = w * g[None, :]
+    I = dR * (f.sum(axis=1) - 0.5 * (f[:, 0] + f[:, -1]))
+    out[ok] = mass / k * np.abs(I) ** 2
+    return out
+
+
 def init_wave_packet(R, delta, R0, k0, phi_r, j, x):
     """psi(R_i, r_j, theta_k) = g(R_i) phi(r_j) sqrt(j+1/2) P_j(x_k) as interleaved
     (2 n1, n2, ntheta) doubles (H3/InitWavePacket.m:13-31), returned flat."""
@@ -458,15 +507,28 @@ class JKProblem:
     coriolis: bool = True
     packet: Optional[dict] = None  # R0, delta, k0, e_v, mu of the initial packet
 
-    def eta_sq_J(self, J: int) -> np.ndarray:
-        """|eta(E)|^2 of the packet measured from the centrifugal energy
-        J(J+1) / (2 mu R0^2) at its centre: a plane-wave component k at R0 has
-        total energy e_v + k^2/2mu + J(J+1)/(2 mu R0^2).  The plain |eta(E)|^2
-        (base.eta_sq) ignores the shift, which at the top of the energy window
-        can push P^J(E) of large J above 1.  Zero where E is below the shift."""
+    def eta_sq_J(self, J: int, kind: str = "partial_wave") -> np.ndarray:
+        """|eta_J(E)|^2 of the packet in partial wave J (its initial state has
+        j0 = 0, so the orbital angular momentum is l = J).
+
+        kind = "partial_wave": the projection onto the energy-normalised
+        incoming wave of the centrifugal potential J(J+1)/2mu R^2
+        (eta_sq_partial_wave); equal to base.eta_sq at J = 0.
+        kind = "wkb": its leading WKB term, the plane-wave |eta|^2 taken at the
+        kinetic energy E - e_v - J(J+1)/(2 mu R0^2) left at the packet centre
+        (zero below that shift).
+
+        The plane-wave |eta(E)|^2 of J = 0 (base.eta_sq) ignores the
+        centrifugal energy the packet starts with; at the top of the energy
+        window it then overstates the incoming flux's deficit and lets P^J(E)
+        of large J exceed 1."""
         pk = self.packet
-        shift = J * (J + 1.0) / (2.0 * pk["mu"] * pk["R0"] ** 2)
-        Ek = np.asarray(self.base.energies, dtype=np.float64) - pk["e_v"] - shift
+        Ek = np.asarray(self.base.energies, dtype=np.float64) - pk["e_v"]
+        if kind == "partial_wave":
+            return eta_sq_partial_wave(pk["delta"], pk["R0"], pk["k0"], pk["mu"], Ek, J)
+        if kind != "wkb":
+            raise ValueError("kind must be 'partial_wave' or 'wkb'")
+        Ek = Ek - J * (J + 1.0) / (2.0 * pk["mu"] * pk["R0"] ** 2)
         out = np.zeros(Ek.size)
         ok = Ek > 0
         out[ok] = eta_sq(pk["delta"], pk["k0"], pk["mu"], Ek[ok])

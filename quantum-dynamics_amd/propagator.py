@@ -28,10 +28,11 @@ VARIANTS = {"cpu": _abi.QD_VARIANT_CPU, "cuda": _abi.QD_VARIANT_CUDA}
 
 
 def make_options(precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True,
+                 diagnostics_every=0):
     return _abi.qd_options(PRECISIONS[precision], PIPELINES[pipeline], VARIANTS[variant], flux_block,
                            crp_every, use_graph, device, profile, theta_split, autotune,
-                           0 if fuse_theta else -1)
+                           0 if fuse_theta else -1, diagnostics_every)
 
 
 def _field(d, name, required=True):
@@ -123,12 +124,12 @@ class Propagator:
 
     def __init__(self, problem: Problem, precision="fp64", pipeline="fused", variant="cpu",
                  flux_block=0, crp_every=0, device=0, profile=0, theta_split=0, use_graph=0, autotune=0,
-                 fuse_theta=True):
+                 fuse_theta=True, diagnostics_every=0):
         self.lib = _abi.load_library()
         self.problem = problem
         self._st = _abi.ProblemStruct(problem)
         self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile,
-                                 theta_split, use_graph, autotune, fuse_theta)
+                                 theta_split, use_graph, autotune, fuse_theta, diagnostics_every)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create(C.byref(self._st.s), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -164,6 +165,14 @@ class Propagator:
         v = C.c_double()
         _abi.check(self.lib.qd_get_norm(self.h, C.byref(v)))
         return v.value
+
+    def diagnostics(self) -> dict:
+        """The reference's per-step diagnostics of the last step that evaluated
+        them (options diagnostics_every; qd_get_diagnostics)."""
+        v = (C.c_double * 5)()
+        _abi.check(self.lib.qd_get_diagnostics(self.h, v))
+        return {"steps": int(v[0]), "module": v[1], "e_kin": v[2], "e_rot": v[3], "e_pot": v[4],
+                "e_tot": v[2] + v[3] + v[4]}
 
     def steps(self) -> int:
         v = C.c_int32()
@@ -264,9 +273,17 @@ class JKPropagator(Propagator):
     """Handle over a batch of (J, K) blocks (qd_create_jk)."""
 
     def __init__(self, jk, precision="fp64", pipeline="fused", variant="cpu", flux_block=0, crp_every=0,
-                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True):
+                 device=0, profile=0, theta_split=0, use_graph=0, autotune=0, fuse_theta=True,
+                 eta="partial_wave", diagnostics_every=0):
+        """eta: the |eta(E)|^2 that normalises each J's flux (qd_jk_block.eta_sq):
+        "partial_wave" (default) the packet's projection onto the incoming wave
+        of partial wave J (JKProblem.eta_sq_J), "wkb" its leading WKB term, or
+        "plane" the J = 0 plane-wave |eta|^2 of base.eta_sq for every J."""
+        if eta not in ("plane", "partial_wave", "wkb"):
+            raise ValueError("eta must be 'plane', 'partial_wave' or 'wkb'")
         self.lib = _abi.load_library()
         self.jk = jk
+        self.eta = eta
         base = jk.base
         self.problem = base
         self._st = _abi.ProblemStruct(base)
@@ -274,6 +291,10 @@ class JKPropagator(Propagator):
         blocks = (_abi.qd_jk_block * len(jk.blocks))()
         n12 = base.n1 * base.n2
         psi = np.ascontiguousarray(jk.psi)
+        self._eta = {}
+        if eta != "plane" and base.calculate_crp:
+            for J in sorted({b.J for b in jk.blocks}):
+                self._eta[J] = np.ascontiguousarray(jk.eta_sq_J(J, kind=eta), dtype=np.float64)
         for i, b in enumerate(jk.blocks):
             # a view (the library copies psi to the device while creating the handle)
             z = psi[b.offset:b.offset + n12 * b.n].view(np.float64)
@@ -282,6 +303,8 @@ class JKPropagator(Propagator):
             blocks[i].K = b.K
             blocks[i].pot = jk.pot[b.K].ctypes.data_as(C.POINTER(C.c_double))
             blocks[i].psi = z.ctypes.data_as(C.POINTER(C.c_double))
+            if b.J in self._eta:
+                blocks[i].eta_sq = self._eta[b.J].ctypes.data_as(C.POINTER(C.c_double))
         self._blocks = blocks
         q = _abi.qd_jk_problem()
         q.base = self._st.s
@@ -291,7 +314,7 @@ class JKPropagator(Propagator):
         q.blocks = C.cast(blocks, C.POINTER(_abi.qd_jk_block))
         self._q = q
         self._opt = make_options(precision, pipeline, variant, flux_block, crp_every, device, profile, theta_split,
-                                 use_graph, autotune, fuse_theta)
+                                 use_graph, autotune, fuse_theta, diagnostics_every)
         h = C.c_void_p()
         _abi.check(self.lib.qd_create_jk(C.byref(q), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -315,23 +338,11 @@ class JKPropagator(Propagator):
         _abi.check(self.lib.qd_get_crp_blocks(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
-    def crp_by_J(self, eta: str = "plane") -> dict:
-        """P^J(E) = sum over the K blocks of J.  eta = "plane" normalises by
-        the plane-wave |eta(E)|^2 of the packet (base.eta_sq, as for J = 0);
-        eta = "centrifugal" by |eta|^2 measured from the centrifugal energy at
-        the packet centre (JKProblem.eta_sq_J)."""
-        if eta not in ("plane", "centrifugal"):
-            raise ValueError("eta must be 'plane' or 'centrifugal'")
+    def crp_by_J(self) -> dict:
+        """P^J(E) = sum over the K blocks of J, each block's flux normalised by
+        the |eta(E)|^2 chosen at creation (`eta`)."""
         pb = self.crp_blocks()
         out = {}
         for i, b in enumerate(self.jk.blocks):
             out[b.J] = out.get(b.J, 0) + pb[i]
-        if eta == "centrifugal":
-            e0 = np.asarray(self.jk.base.eta_sq, dtype=np.float64)
-            for J in out:
-                eJ = self.jk.eta_sq_J(J)
-                f = np.zeros_like(e0)
-                ok = eJ > 0
-                f[ok] = e0[ok] / eJ[ok]
-                out[J] = out[J] * f
         return out

@@ -32,7 +32,8 @@ def test_abi_struct_layout_matches_header(qd):
     assert C.sizeof(abi.qd_angle) == 32
     assert C.sizeof(abi.qd_time) == 16
     assert C.sizeof(abi.qd_crp) == 40
-    assert C.sizeof(abi.qd_options) == 44
+    assert C.sizeof(abi.qd_options) == 48
+    assert C.sizeof(abi.qd_jk_block) == 32
     assert C.sizeof(abi.qd_problem) == 32 + 32 + 32 + 8 + 8 + 16 + 8 + 8 + 40
 
 

@@ -32,7 +32,7 @@ def test_jk_fp64_matches_numpy_oracle(qd, jkcase, coriolis):
     steps = jk.base.total_steps
     print("case: jk fp64 coriolis", coriolis, "blocks", len(jk.blocks), flush=True)
     psi_o, PJ, _ = jk_oracle.propagate(jk, steps)
-    with qd.propagator.JKPropagator(jk, precision="fp64") as prop:
+    with qd.propagator.JKPropagator(jk, precision="fp64", eta="plane") as prop:
         prop.propagate(steps)
         psi_g = prop.psi().view(np.complex128)
         PJg = prop.crp_by_J()
@@ -53,7 +53,7 @@ def test_jk_block_zero_equals_single_block_path(qd, jkcase):
     p.legendre = pb.legendre_p2(p.m, p.x)
     p.pot = jk.pot[0]
     p.psi = jk.psi.view(np.float64).copy()
-    with qd.propagator.JKPropagator(jk, precision="fp64") as a:
+    with qd.propagator.JKPropagator(jk, precision="fp64", eta="plane") as a:
         a.propagate(steps)
         pa, ca = a.psi(), a.crp()
     with qd.Propagator(p, precision="fp64") as b:

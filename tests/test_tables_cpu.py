@@ -49,15 +49,16 @@ def test_centrifugal_eta_shift(qd):
     base = pb.config(0, nR=32, nr=24, ntheta=6, steps=2)
     jk = pb.make_jk_problem(base, pb.H_H2, [0, 6], jmax=5)
     # J = 0: no shift, the plain |eta|^2 of the packet
-    assert np.allclose(jk.eta_sq_J(0), jk.base.eta_sq, rtol=1e-12, atol=0)
+    assert np.allclose(jk.eta_sq_J(0, kind="wkb"), jk.base.eta_sq, rtol=1e-12, atol=0)
     pk = jk.packet
     J = 6
     shift = J * (J + 1) / (2 * pk["mu"] * pk["R0"] ** 2)
     Ek = jk.base.energies - pk["e_v"] - shift
     want = np.where(Ek > 0, pb.eta_sq(pk["delta"], pk["k0"], pk["mu"], np.maximum(Ek, 1e-300)), 0.0)
-    assert np.allclose(jk.eta_sq_J(J), want, rtol=1e-12, atol=0)
+    assert np.allclose(jk.eta_sq_J(J, kind="wkb"), want, rtol=1e-12, atol=0)
     # the shifted |eta|^2 is the plain one evaluated shift lower in energy: its peak moves up by the shift
-    assert jk.base.energies[np.argmax(jk.eta_sq_J(J))] >= jk.base.energies[np.argmax(jk.base.eta_sq)]
+    assert (jk.base.energies[np.argmax(jk.eta_sq_J(J, kind="wkb"))]
+            >= jk.base.energies[np.argmax(jk.base.eta_sq)])
 
 
 @pytest.fixture(scope="module")
@@ -84,3 +85,47 @@ def test_bundle_packing(pack_harness, Jmax, jmax, n2, nl, min_fill):
     groups = sum(min(J, j) + 1 for J in range(Jmax + 1) for j in range(jmax + 1))
     assert lines == groups * n2
     assert fill >= min_fill, fill
+
+
+@pytest.fixture(scope="module")
+def records_harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("records") / "records")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-o", exe,
+                    os.path.join(HERE, "cpu_harness", "records_main.cpp")], check=True)
+    return exe
+
+
+# (n1, n2, jmax, coriolis, Js): the batched workloads and the GPU tests' batches
+RECORD_WORKLOADS = [
+    (256, 192, 60, 1, list(range(21))),               # throughput workload: J = 0..20, 231 blocks
+    (256, 192, 60, 1, list(range(41))),               # every J of the learned-interpolation system
+    (256, 192, 60, 1, list(range(0, 41, 4))),         # its propagated J
+    (256, 192, 60, 1, [1, 9, 17, 25, 33]),            # its held-out J
+    (256, 192, 60, 1, [0, 10, 20, 40]),               # the full-length accuracy batch
+    (256, 192, 60, 0, [0, 10, 20]),                   # centrifugal-sudden (no mixing matrices)
+    (48, 32, 7, 1, [0, 1, 3]),                        # oracle-parity batch
+    (64, 32, 11, 1, [0, 4, 9]),                       # prefetch-mixing accuracy batch
+    (64, 16, 35, 1, [33]),                            # 34-line groups (48-line bundles)
+    (64, 48, 8, 1, list(range(33))),                  # H+H2 learned-interpolation batch
+]
+
+
+@pytest.mark.parametrize("n1,n2,jmax,coriolis,Js", RECORD_WORKLOADS)
+def test_work_records_stay_in_bounds(records_harness, n1, n2, jmax, coriolis, Js):
+    """Every index the mixing R pass and the fused theta kernel form from their
+    host-built records lies inside the arrays they read and write, for every
+    batched workload (qd_records.hpp: the library's own checks, then an
+    independent walk over lines, matrix reads and tiles)."""
+    r = subprocess.run([records_harness, str(n1), str(n2), str(jmax), str(coriolis)] + [str(J) for J in Js],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5])
+def test_work_record_checks_catch_planted_mistakes(records_harness, plant):
+    """A line past the FBR array, a unit past its bundle, a D_R row past the
+    table, a tile past the grid and an uncovered tile are each reported."""
+    env = dict(os.environ, QD_PLANT=str(plant))
+    r = subprocess.run([records_harness, "64", "32", "11", "1", "0", "4", "9"], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0 and r.stdout.startswith("fail"), r.stdout
