@@ -178,6 +178,18 @@ int qd_destroy(qd_handle *h);
  * flux accumulators are kept. */
 int qd_set_psi(qd_handle *h, const double *psi_in);
 
+/* Checkpoint of the whole propagation state -- the wavefunction as the handle
+ * holds it (psi, or the FBR array of the fused theta step), the energy-resolved
+ * flux accumulators phi(E) and dphi(E) with the surface slices of the open time
+ * block, the step counts and the pending first half step -- into a caller
+ * buffer of qd_checkpoint_size bytes.  Loading it into a handle created from
+ * the same problem and options continues the propagation bit for bit as if it
+ * had not stopped (the reference keeps these only in memory, timeEvol.C:762-805,
+ * and restarts them on every call). */
+int qd_checkpoint_size(qd_handle *h, int64_t *bytes);
+int qd_checkpoint_save(qd_handle *h, void *buf, int64_t bytes);
+int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes);
+
 /* ---- J > 0: body-fixed (J, K) blocks ----------------------------------
  * No counterpart in the reference, which propagates J = 0 only (the rotational
  * term of common/timeEvol.C:487 is l(l+1)(1/2mu1R^2 + 1/2mu2r^2)).  Block
@@ -236,6 +248,46 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);
+
+/* ---- set-up -------------------------------------------------------------------
+ * What the reference's MATLAB drivers compute before they call the propagator,
+ * restated in C++ so that a caller can take packet parameters instead of a
+ * precomputed psi.  Host code only: these need no GPU. */
+
+/* r[n] = linspace(lo, hi, n) and *dr = r[1] - r[0] (the drivers' grids, H3/main.m:45-60). */
+int qd_make_radial(int32_t n, double lo, double hi, double *r, double *dr);
+/* theta.x and theta.w (n Gauss-Legendre points, ascending; common/GaussLegendre.F:4-38,
+ * GaussLegendre2.m:4-32) and theta.legendre ((m+1) x n, legendre[l + (m+1)*k] = P_l(x_k);
+ * LegendreP2.m:4-23 transposed as H3/main.m:101-104 does).  legendre may be NULL. */
+int qd_make_theta(int32_t n, int32_t m, double *x, double *w, double *legendre);
+/* n_dividing_surface = int32((rd - rmin) / dr), rounding half away from zero as
+ * MATLAB's int32() does (H3/main.m:77). */
+int qd_dividing_surface(double rd, double rmin, double dr, int32_t *index);
+
+/* The initial state of InitWavePacket.m (H3/InitWavePacket.m:2-31). */
+typedef struct qd_packet {
+  double R0, delta, k0;   /* Gaussian in R: centre, width, mean momentum (r1.r0, r1.delta, r1.k0) */
+  int32_t v0, j0;         /* vibrational level and rotation of the diatomic */
+  int32_t J, K;           /* total angular momentum and its body-fixed projection; J = 0, K = 0
+                             is the reference's problem */
+  const double *vdiat;    /* diatomic potential on r2.r, n2 values (Eh; H3/H2PES.m) */
+} qd_packet;
+
+/* psi (2*n1*n2*ntheta interleaved) = g(R_i) phi(r_j) Theta(x_k) with
+ *   g(R) = (pi delta^2)^(-1/4) exp(-(R - R0)^2 / (2 delta^2) - i k0 R),
+ *   phi  = eigenstate v0 of the diatomic in rotation j0 on the r grid, by the
+ *          radial sine DVR of H3/H2VibRotWaveFunction.m:19-46 (indexed from 1;
+ *          normalised to sum phi^2 dr = 1, its largest value positive),
+ *   Theta = sqrt(j0 + 1/2) P_j0(x_k) for J = 0 on theta.x (the reference's
+ *          Gauss-Legendre points), and the normalised associated Legendre
+ *          Pt_j0^K(x_k) for J > 0 on the block's points (qd_jk_theta_grid);
+ * *e_diatomic = e_v (Eh), and eta_sq[e] = |eta(energies[e] - e_v)|^2, the flux
+ * normalisation: common/EtaSq.m:4-18 for J = 0, and for J > 0 the packet's
+ * projection onto the energy-normalised incoming wave of orbital angular
+ * momentum l = J (which reduces to EtaSq.m at l = 0; needs j0 = 0, where l = J).
+ * eta_sq may be NULL when n_energies = 0; psi may be NULL to get only e_v and eta. */
+int qd_init_packet(const qd_radial *r1, const qd_radial *r2, const qd_angle *theta, const qd_packet *packet,
+                   int32_t n_energies, const double *energies, double *psi, double *e_diatomic, double *eta_sq);
 
 /* Last error message of the calling thread (empty string if none). */
 const char *qd_last_error(void);

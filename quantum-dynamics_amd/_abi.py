@@ -52,6 +52,11 @@ class qd_options(C.Structure):
                 ("diagnostics_every", C.c_int32)]
 
 
+class qd_packet(C.Structure):
+    _fields_ = [("R0", C.c_double), ("delta", C.c_double), ("k0", C.c_double), ("v0", C.c_int32),
+                ("j0", C.c_int32), ("J", C.c_int32), ("K", C.c_int32), ("vdiat", _dp)]
+
+
 class qd_jk_block(C.Structure):
     _fields_ = [("J", C.c_int32), ("K", C.c_int32), ("pot", _dp), ("psi", _dp), ("eta_sq", _dp)]
 

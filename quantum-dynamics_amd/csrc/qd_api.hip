@@ -33,6 +33,7 @@
 #include "qd_dispatch.hpp"
 #include "qd_pack.hpp"
 #include "qd_records.hpp"
+#include "qd_setup.hpp"
 #include "qd_tables.hpp"
 
 using namespace qd;
@@ -2009,6 +2010,78 @@ int qd_jk_theta_grid(int32_t K, int32_t jmax, double *x, double *w) {
   return QD_OK;
 }
 
+int qd_make_radial(int32_t n, double lo, double hi, double *r, double *dr) {
+  if (!r || !dr) return fail(QD_ERR_NULL, "NULL argument");
+  if (n < 2 || !(hi > lo)) return fail(QD_ERR_ARG, "need n >= 2 and hi > lo");
+  const double step = (hi - lo) / (double)(n - 1);  // linspace: lo + i step, the last point hi exactly
+  for (int i = 0; i < n; i++) r[i] = i == n - 1 ? hi : lo + i * step;
+  *dr = r[1] - r[0];
+  return QD_OK;
+}
+
+int qd_make_theta(int32_t n, int32_t m, double *x, double *w, double *legendre) {
+  if (!x || !w) return fail(QD_ERR_NULL, "NULL argument");
+  if (n < 1 || m < 0) return fail(QD_ERR_ARG, "need n >= 1 and m >= 0");
+  qds::gauss_legendre(n, x, w);
+  if (legendre) qds::legendre_p2(m, x, n, legendre);
+  return QD_OK;
+}
+
+int qd_dividing_surface(double rd, double rmin, double dr, int32_t *index) {
+  if (!index) return fail(QD_ERR_NULL, "NULL argument");
+  if (!(dr > 0)) return fail(QD_ERR_ARG, "dr must be positive");
+  *index = qds::dividing_surface(rd, rmin, dr);
+  return QD_OK;
+}
+
+int qd_init_packet(const qd_radial *r1, const qd_radial *r2, const qd_angle *theta, const qd_packet *pk,
+                   int32_t n_energies, const double *energies, double *psi, double *e_diatomic, double *eta_sq) {
+  if (!r1 || !r2 || !theta || !pk || !e_diatomic) return fail(QD_ERR_NULL, "NULL argument");
+  if (!r1->r || !r2->r || !pk->vdiat || (psi && !theta->x)) return fail(QD_ERR_NULL, "a grid or vdiat is NULL");
+  if (n_energies > 0 && (!energies || !eta_sq)) return fail(QD_ERR_NULL, "energies or eta_sq is NULL");
+  if (r1->n < 1 || r2->n < 1 || (psi && theta->n < 1)) return fail(QD_ERR_ARG, "grid sizes must be positive");
+  if (!(r1->mass > 0) || !(r2->mass > 0) || !(r2->dr > 0) || !(pk->delta > 0))
+    return fail(QD_ERR_ARG, "masses, dr and delta must be positive");
+  if (pk->v0 < 0 || pk->v0 >= r2->n || pk->j0 < 0) return fail(QD_ERR_ARG, "need 0 <= v0 < r2.n and j0 >= 0");
+  if (pk->J < 0 || pk->K < 0 || pk->K > pk->J || pk->K > pk->j0)
+    return fail(QD_ERR_ARG, "need 0 <= K <= min(J, j0) for the packet's block");
+  if (pk->J > 0 && pk->j0 != 0 && n_energies > 0)
+    return fail(QD_ERR_ARG, "the partial-wave normalisation for J > 0 needs j0 = 0 (then l = J)");
+  const int n1 = r1->n, n2 = r2->n;
+  std::vector<double> phi(n2);
+  double ev = 0;
+  qds::diatomic_state(r2->r, n2, r2->dr, r2->mass, pk->vdiat, pk->j0, pk->v0, &ev, phi.data());
+  *e_diatomic = ev;
+  if (psi) {
+    const int nt = theta->n;
+    std::vector<double> th(nt);
+    if (pk->J == 0) {  // sqrt(j0 + 1/2) P_j0(x) by LegendreP.m's recursion (InitWavePacket.m:21-22)
+      std::vector<double> P((size_t)(pk->j0 + 1) * nt);
+      qds::legendre_p2(pk->j0, theta->x, nt, P.data());
+      for (int k = 0; k < nt; k++) th[k] = std::sqrt(pk->j0 + 0.5) * P[(size_t)(pk->j0 + 1) * k + pk->j0];
+    } else {
+      for (int k = 0; k < nt; k++) th[k] = qds::assoc_legendre_norm(pk->j0, pk->K, theta->x[k]);
+    }
+    std::vector<std::complex<double>> g(n1);
+    for (int i = 0; i < n1; i++) g[i] = qds::packet_g(r1->r[i], pk->R0, pk->delta, pk->k0);
+    for (int k = 0; k < nt; k++)
+      for (int j = 0; j < n2; j++) {
+        double *dst = psi + 2 * ((size_t)n1 * (j + (size_t)n2 * k));
+        const double f = phi[j] * th[k];
+        for (int i = 0; i < n1; i++) {
+          dst[2 * i] = g[i].real() * f;
+          dst[2 * i + 1] = g[i].imag() * f;
+        }
+      }
+  }
+  for (int e = 0; e < n_energies; e++) {
+    const double Et = energies[e] - ev;
+    eta_sq[e] = pk->J == 0 ? qds::eta_sq_plane(pk->delta, pk->k0, r1->mass, Et)
+                           : qds::eta_sq_partial_wave(pk->R0, pk->delta, pk->k0, r1->mass, pk->J, Et);
+  }
+  return QD_OK;
+}
+
 int qd_propagate(qd_handle *h, int32_t n_steps) {
   if (!h) return fail(QD_ERR_NULL, "handle is NULL");
   return propagate_impl(h, n_steps);
@@ -2110,6 +2183,116 @@ int qd_set_psi(qd_handle *h, const double *psi_in) {
   h->dvr_valid = true;
   return QD_OK;
 }
+
+}  // extern "C"
+
+// ---- checkpoints ----
+namespace {
+struct CkptHead {
+  char magic[8];                                   // "QDCKPT1"
+  int64_t npt, nlp, nx, ne, tblock, prec, nb;      // shape of the handle it came from
+  int64_t steps, acc_steps, slot, state_fbr, pending_prephase, bytes;
+};
+int64_t ckpt_bytes(const qd_handle *h) {
+  const int64_t cs = h->prec == QD_PREC_FP64 ? 16 : 8;
+  int64_t b = sizeof(CkptHead) + cs * (int64_t)std::max(h->npt, h->nlp);
+  if (h->do_crp) b += 2 * 16 * (int64_t)h->nx * h->ne + 2 * 16 * (int64_t)h->nx * h->tblock;
+  return b;
+}
+void ckpt_head(const qd_handle *h, CkptHead &c) {
+  memset(&c, 0, sizeof c);
+  memcpy(c.magic, "QDCKPT1", 8);
+  c.npt = (int64_t)h->npt;
+  c.nlp = (int64_t)h->nlp;
+  c.nx = h->do_crp ? (int64_t)h->nx : 0;
+  c.ne = h->ne;
+  c.tblock = h->tblock;
+  c.prec = h->prec;
+  c.nb = h->nb;
+  c.bytes = ckpt_bytes(h);
+}
+}  // namespace
+
+extern "C" {
+
+int qd_checkpoint_size(qd_handle *h, int64_t *bytes) {
+  if (!h || !bytes) return fail(QD_ERR_NULL, "NULL argument");
+  *bytes = ckpt_bytes(h);
+  return QD_OK;
+}
+
+int qd_checkpoint_save(qd_handle *h, void *buf, int64_t bytes) {
+  if (!h || !buf) return fail(QD_ERR_NULL, "NULL argument");
+  if (bytes < ckpt_bytes(h)) return fail(QD_ERR_ARG, "checkpoint buffer smaller than qd_checkpoint_size");
+  QD_HIP(hipSetDevice(h->dev));
+  CkptHead c;
+  ckpt_head(h, c);
+  c.steps = h->steps;
+  c.acc_steps = h->acc_steps;
+  c.slot = h->slot;
+  c.state_fbr = h->state_fbr ? 1 : 0;
+  c.pending_prephase = h->pending_prephase ? 1 : 0;
+  char *p = static_cast<char *>(buf);
+  memcpy(p, &c, sizeof c);
+  p += sizeof c;
+  const bool f64 = h->prec == QD_PREC_FP64;
+  const size_t cs = f64 ? 16 : 8;
+  // the wavefunction in the form the handle propagates it, raw (bit-exact)
+  const void *src = h->state_fbr ? (f64 ? (const void *)h->lpsi64 : (const void *)h->lpsi32)
+                                 : (f64 ? (const void *)h->psi64 : (const void *)h->psi32);
+  const size_t n = h->state_fbr ? h->nlp : h->npt;
+  QD_HIP(hipMemcpy(p, src, cs * n, hipMemcpyDeviceToHost));
+  p += cs * std::max(h->npt, h->nlp);
+  if (h->do_crp) {
+    const size_t acc = sizeof(double2) * h->nx * h->ne, ring = sizeof(double2) * h->nx * h->tblock;
+    QD_HIP(hipMemcpy(p, h->fai, acc, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(p + acc, h->dfai, acc, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(p + 2 * acc, h->bpsi, ring, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(p + 2 * acc + ring, h->bdpsi, ring, hipMemcpyDeviceToHost));
+  }
+  return QD_OK;
+}
+
+int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes) {
+  if (!h || !buf) return fail(QD_ERR_NULL, "NULL argument");
+  if (bytes < (int64_t)sizeof(CkptHead)) return fail(QD_ERR_ARG, "checkpoint too short");
+  CkptHead c, want;
+  memcpy(&c, buf, sizeof c);
+  ckpt_head(h, want);
+  if (memcmp(c.magic, want.magic, 8) || c.npt != want.npt || c.nlp != want.nlp || c.nx != want.nx ||
+      c.ne != want.ne || c.tblock != want.tblock || c.prec != want.prec || c.nb != want.nb || c.bytes != want.bytes)
+    return fail(QD_ERR_ARG, "checkpoint does not belong to a handle of this problem and options");
+  if (bytes < c.bytes) return fail(QD_ERR_ARG, "checkpoint buffer shorter than its header says");
+  if (c.state_fbr && !h->fuse_theta) return fail(QD_ERR_ARG, "checkpoint holds the fused theta state");
+  QD_HIP(hipSetDevice(h->dev));
+  QD_HIP(hipStreamSynchronize(h->stream));
+  const char *p = static_cast<const char *>(buf) + sizeof c;
+  const bool f64 = h->prec == QD_PREC_FP64;
+  const size_t cs = f64 ? 16 : 8;
+  void *dst = c.state_fbr ? (f64 ? (void *)h->lpsi64 : (void *)h->lpsi32) : (f64 ? (void *)h->psi64 : (void *)h->psi32);
+  QD_HIP(hipMemcpy(dst, p, cs * (c.state_fbr ? h->nlp : h->npt), hipMemcpyHostToDevice));
+  p += cs * std::max(h->npt, h->nlp);
+  if (h->do_crp) {
+    const size_t acc = sizeof(double2) * h->nx * h->ne, ring = sizeof(double2) * h->nx * h->tblock;
+    QD_HIP(hipMemcpy(h->fai, p, acc, hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(h->dfai, p + acc, acc, hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(h->bpsi, p + 2 * acc, ring, hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(h->bdpsi, p + 2 * acc + ring, ring, hipMemcpyHostToDevice));
+    const long long a = c.acc_steps;
+    QD_HIP(hipMemcpy(h->d_acc_steps, &a, sizeof a, hipMemcpyHostToDevice));
+  }
+  h->steps = (int32_t)c.steps;
+  h->acc_steps = c.acc_steps;
+  h->slot = (int)c.slot;
+  h->state_fbr = c.state_fbr != 0;
+  h->dvr_valid = !h->state_fbr;
+  h->pending_prephase = c.pending_prephase != 0;
+  return QD_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int qd_debug_get_fbr(qd_handle *h, double *out) {
   if (!h || !out) return fail(QD_ERR_NULL, "NULL argument");

@@ -174,6 +174,19 @@ class Propagator:
         return {"steps": int(v[0]), "module": v[1], "e_kin": v[2], "e_rot": v[3], "e_pot": v[4],
                 "e_tot": v[2] + v[3] + v[4]}
 
+    def checkpoint(self) -> np.ndarray:
+        """The whole propagation state (qd_checkpoint_save) as a byte array."""
+        n = C.c_int64()
+        _abi.check(self.lib.qd_checkpoint_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        _abi.check(self.lib.qd_checkpoint_save(self.h, buf.ctypes.data_as(C.c_void_p), C.c_int64(n.value)))
+        return buf
+
+    def restore(self, buf: np.ndarray):
+        """Continue from a checkpoint of a handle of the same problem and options."""
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        _abi.check(self.lib.qd_checkpoint_load(self.h, b.ctypes.data_as(C.c_void_p), C.c_int64(b.size)))
+
     def steps(self) -> int:
         v = C.c_int32()
         _abi.check(self.lib.qd_get_steps(self.h, C.byref(v)))
@@ -255,6 +268,62 @@ def _debug_methods(cls):
 _debug_methods(Propagator)
 Propagator.n_points = property(lambda self: self.problem.grid_points)
 Propagator.n_fbr_points = property(lambda self: self.problem.n1 * self.problem.n2 * (self.problem.m + 1))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_radial(n: int, lo: float, hi: float):
+    """(r, dr) of qd_make_radial: linspace(lo, hi, n)."""
+    lib = _abi.load_library()
+    r = np.empty(int(n))
+    dr = C.c_double()
+    _abi.check(lib.qd_make_radial(int(n), C.c_double(lo), C.c_double(hi), _dptr(r), C.byref(dr)))
+    return r, dr.value
+
+
+def make_theta(n: int, m: int):
+    """(x, w, legendre) of qd_make_theta; legendre as the (m+1, n) matrix P_l(x_k)."""
+    lib = _abi.load_library()
+    x, w = np.empty(n), np.empty(n)
+    leg = np.empty((n, m + 1))          # column k holds P_0..P_m at x_k (l fastest)
+    _abi.check(lib.qd_make_theta(int(n), int(m), _dptr(x), _dptr(w), _dptr(leg)))
+    return x, w, np.ascontiguousarray(leg.T)
+
+
+def dividing_surface(rd: float, rmin: float, dr: float) -> int:
+    lib = _abi.load_library()
+    v = C.c_int32()
+    _abi.check(lib.qd_dividing_surface(C.c_double(rd), C.c_double(rmin), C.c_double(dr), C.byref(v)))
+    return int(v.value)
+
+
+def init_packet(R, r, mass1, mass2, vdiat, x=None, R0=8.0, delta=0.3, k0=0.0, v0=0, j0=0, J=0, K=0,
+                energies=None, want_psi=True):
+    """qd_init_packet: (psi interleaved or None, e_v, eta_sq or None) for the
+    packet g(R) phi_{v0 j0}(r) Theta(x) on the grids R, r, x."""
+    lib = _abi.load_library()
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    vd = np.ascontiguousarray(vdiat, dtype=np.float64)
+    r1 = _abi.qd_radial(R.size, _dptr(R), float(R[1] - R[0]), float(mass1))
+    r2 = _abi.qd_radial(r.size, _dptr(r), float(r[1] - r[0]), float(mass2))
+    th = _abi.qd_angle()
+    xs = None
+    if x is not None:
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        th.n = xs.size
+        th.x = _dptr(xs)
+    pk = _abi.qd_packet(float(R0), float(delta), float(k0), int(v0), int(j0), int(J), int(K), _dptr(vd))
+    E = np.ascontiguousarray(energies if energies is not None else np.zeros(0), dtype=np.float64)
+    eta = np.empty(E.size)
+    psi = np.empty(2 * R.size * r.size * xs.size) if (want_psi and xs is not None) else None
+    ev = C.c_double()
+    _abi.check(lib.qd_init_packet(C.byref(r1), C.byref(r2), C.byref(th), C.byref(pk), int(E.size),
+                                  _dptr(E) if E.size else None, _dptr(psi) if psi is not None else None,
+                                  C.byref(ev), _dptr(eta) if E.size else None))
+    return psi, ev.value, (eta if E.size else None)
 
 
 def jk_theta_grid(K: int, jmax: int):

@@ -13,6 +13,10 @@ Fixtures (inputs and outputs only, small):
                      several n, and the tabulated 32-digit half-grids of
                      GaussLegendre/<n>-LegendreGauss.grids for n = 40, 80, 120, 300
   ksg_leps.npz       OHClKSGPES (OHCl/OHClKSG.F) at seeded geometries
+  hcl_ksg_levels.npz the H-Cl curve of OHClKSG.F (O far away, as OHCl/HClPES.m
+                     places it) on a 512-point grid over 1..40 a0, the H-Cl
+                     reduced mass of OHCl/HClVibRotTest.m:7-20, and the 25
+                     vibrational levels that script records (:33-57)
   stencils.json      central-difference coefficient table of
                      common-cuda/gradients2.cu:37-581 (half-widths 1..30)
 """
@@ -110,6 +114,19 @@ def main():
         geo.append((a, b, c))
         vals.append(v.value)
     np.savez(os.path.join(HERE, "ksg_leps.npz"), geo=np.array(geo), v=np.array(vals))
+
+    # --- HCl levels: the KSG H-Cl curve and the levels OHCl/HClVibRotTest.m records ---
+    mass_au = 1.822888484929367e+03
+    m_h, m_cl = 1.0079 * mass_au, 35.453 * mass_au
+    r = np.linspace(1.0, 40.0, 512)
+    v_hcl = np.empty(r.size)
+    for i, x in enumerate(r):
+        v = C.c_double()
+        lib.ohclksgpes_(D(float(np.sqrt(x * x + 100.0 * 100.0))), D(100.0), D(float(x)), C.byref(v))
+        v_hcl[i] = v.value
+    script = open(os.path.join(REF, "OHCl", "HClVibRotTest.m")).read()
+    levels = np.array([float(t) for t in script.split("return", 1)[1].split()])
+    np.savez(os.path.join(HERE, "hcl_ksg_levels.npz"), r=r, v=v_hcl, mu=m_h * m_cl / (m_h + m_cl), levels=levels)
 
     # --- gradients2.cu coefficient table (decimal literals, half-widths 1..30) ---
     src = open(os.path.join(REF, "common-cuda", "gradients2.cu")).read()

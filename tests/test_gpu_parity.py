@@ -188,3 +188,29 @@ def test_fused_theta_step_matches_unfused(qd, kind):
           % (nm_d, n_d, nm_u, n_u, nm_f, n_f))
     assert abs(nm_f - nm_d) <= 2 * abs(nm_u - nm_d) + 1e-6 * nm_d
     assert abs(n_f - n_d) <= 2 * abs(n_u - n_d) + 1e-6 * n_d
+
+
+@pytest.fixture(scope="module")
+def wide_r(qd):
+    """H+H2 on a finer r grid (192 points over 0.4..6 a0) with the surface at
+    1.6 a0 (row 41), inside the H2 wavefunction, so that stencils up to 61
+    points (30 rows each side) fit and the surface carries flux from the first
+    steps (|P| ~ 0.2 after 30 steps, differing by 1 % between 3 and 31 points)."""
+    return qd.problem.config(0, nr=192, r_range=(0.4, 6.0), rs=1.6, steps=30)
+
+
+@pytest.mark.parametrize("npoints", [3, 5, 7, 9, 31, 61])
+@pytest.mark.parametrize("variant", ["cpu", "cuda"])
+def test_flux_stencils_match_oracle(qd, oracle, wide_r, npoints, variant):
+    """The surface gradient with every stencil width the reference propagators
+    take (3..11 points in Gradient3.F:15-26; odd 3..61 in gradients2.cu:566-569;
+    31 in HO2/DMBEIV/main.m:105), in both propagators' conventions, FP64
+    against the oracle."""
+    p = wide_r.copy()
+    p.n_gradient_points = npoints
+    print("case: stencil", npoints, variant, flush=True)
+    psi_g, crp_g = qd.run(p, precision="fp64", variant=variant)
+    psi_o, crp_o, _, _ = oracle.time_evolution(p, variant=1 if variant == "cuda" else 0)
+    assert rel_psi(psi_g, psi_o) < 1e-10
+    assert crp_err(crp_g, crp_o) < 1e-8
+    assert np.max(np.abs(crp_o)) > 0.1  # the surface sees flux

@@ -260,6 +260,30 @@ def test_static_phase_and_first_half_step(qd, oracle):
     assert np.max(np.abs(psi_c.view(np.complex128) - ref_c)) < 1e-11
 
 
+def test_dump_closed_form_in_both_orders(qd, oracle):
+    """Infinite masses with the absorbers on: psi_n = psi_0 e^{iV dt/2}
+    e^{-iV n dt} (d1 d2)^n in both propagators -- the CPU one dumps after each
+    step (timeEvol.C:667, dump_psi :713-735), the CUDA one before it
+    (evolutionCUDA2.cu:503) -- and, after n steps, both have applied n dumps.
+    V dt stays below pi so the first half phase is e^{iV dt/2} in both."""
+    p = _tiny_problem(qd, mass=1e30, dump=True, steps=9, dt=5.0)
+    rng = np.random.default_rng(5)
+    p.pot = rng.uniform(0.0, 0.6, p.grid_points)
+    p.dump1 = rng.uniform(0.5, 1.0, p.n1)
+    p.dump2 = rng.uniform(0.5, 1.0, p.n2)
+    p.calculate_crp = 0
+    z0 = p.psi.view(np.complex128)
+    d = (p.dump2[:, None] * p.dump1[None, :]).reshape(-1)       # (j, i) -> i + n1 j
+    dd = np.tile(d, p.ntheta)
+    n = p.total_steps
+    ref = z0 * np.exp(0.5j * p.pot * p.dt) * np.exp(-1j * p.pot * p.dt * n) * dd ** n
+    for variant in (0, 1):
+        psi, _, _, _ = oracle.time_evolution(p, variant=variant)
+        assert np.max(np.abs(psi.view(np.complex128) - ref)) < 1e-12 * np.max(np.abs(ref)), variant
+    # one dump fewer would be off by a factor d: the check sees the count
+    assert np.max(np.abs(ref / dd - ref)) > 1e-3 * np.max(np.abs(ref))
+
+
 def test_flux_accumulation_geometric_series(qd, oracle):
     """Infinite masses and constant V0: psi_s(t_n) = psi_s(0) e^{iV0 dt/2} e^{-iV0 (n+1) dt},
     so fai(E) = G(E) psi_s, G = sum_n dt e^{i (E - V0)(n+1) dt}, and
