@@ -238,8 +238,8 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
  * [9] its unit-size bucket, [10] fused theta step in use, [11] its 16-row
- * tiles per work group, [12] that mixing R pass runs two work groups per CU.
- * vals holds 13 entries. */
+ * tiles per work group, [12] that mixing R pass runs two work groups per CU,
+ * [13] threads per work group of the fused theta step.  vals holds 14 entries. */
 int qd_get_tuning(qd_handle *h, int32_t *vals);
 /* test hook: force one kernel choice the autotuner makes.  what 0: the
  * prefetching FP32 mixing R pass with one (value 0) or two (value 1) work

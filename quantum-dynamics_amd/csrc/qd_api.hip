@@ -232,7 +232,7 @@ struct qd_handle {
   // fused theta step (k_theta_fused): between steps the state is the FBR array
   // holding theta-forward(V psi); psi itself is rebuilt from it when read
   bool fuse_theta = false, state_fbr = false, dvr_valid = true;
-  int fuse_opt = 0, tfu_tpw = 1;
+  int fuse_opt = 0, tfu_tpw = 1, tfu_nth = TMX_THREADS;  // tile-blocks per item, threads per work group
   size_t tfu_lds = 0;
   // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
   // matrices and the V columns, and the work items built from them
@@ -552,12 +552,12 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
   F.r_hi = h->do_crp && rows ? h->surf + h->stencil.h : 0;
   F.items = h->d_tfu_items;
   size_t lds = 0;
-  const void *fn = theta_fused_kernel(h->tmx_ks, &lds);
+  const void *fn = theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
   if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
   if (h->tfu_nitems < 1) return fail(QD_ERR_ARG, "fused theta work items not set up");
   dim3 grid((unsigned)h->tfu_nitems, 1, 1);
   void *args[] = {&F};
-  QD_HIP(hipLaunchKernel(fn, grid, dim3(TMX_THREADS), args, lds, h->stream));
+  QD_HIP(hipLaunchKernel(fn, grid, dim3(h->tfu_nth), args, lds, h->stream));
   QD_HIP(hipGetLastError());
   return QD_OK;
 }
@@ -567,7 +567,7 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
 // about tfu_tpw tile-blocks.  Set-up and autotuning only (a synchronous copy).
 int build_tfu_items(qd_handle *h) {
   const long long ntiles = ((long long)h->n12 + 15) / 16;
-  const std::vector<TfuItem> items = qd::build_tfu_items(h->tfu_groups, ntiles, h->tfu_tpw, TMX_THREADS / 64);
+  const std::vector<TfuItem> items = qd::build_tfu_items(h->tfu_groups, ntiles, h->tfu_tpw, h->tfu_nth / 64);
   const std::string err = check_tfu_items(items, h->tfu_blocks, (long long)h->n12, h->ndcol, h->nfcol, h->nV);
   if (!err.empty()) return fail(QD_ERR_ARG, err);
   if ((int)items.size() > h->tfu_items_cap) {
@@ -873,7 +873,9 @@ int setup_flux(qd_handle *h, const qd_problem *p) {
 }
 
 // Upload a complex table given in FP64: as double2 for the FP64 path, as
-// float4 (hi, lo) pairs for the mixed path (see tab_of in qd_kernels.hpp).
+// float4 (hi, lo) pairs for the mixed path (see tab_of in qd_kernels.hpp),
+// hi rounded to keep each factor's modulus (qdt::round_unit_complex) and
+// lo = the rest.
 int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
   if (h->prec == QD_PREC_FP64) {
     double *d = nullptr;
@@ -883,7 +885,8 @@ int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
     const size_t n = v.size() / 2;
     std::vector<float> f(4 * n);
     for (size_t i = 0; i < n; i++) {
-      const float hx = (float)v[2 * i], hy = (float)v[2 * i + 1];
+      float hx = 0.f, hy = 0.f;
+      qdt::round_unit_complex(v[2 * i], v[2 * i + 1], hx, hy);
       f[4 * i] = hx;
       f[4 * i + 1] = hy;
       f[4 * i + 2] = (float)(v[2 * i] - (double)hx);
@@ -1305,8 +1308,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
       std::vector<float> e(2 * vh.size());
       for (size_t x = 0; x < vh.size(); x++) {
         const double a = -h->dt * vh[x];
-        e[2 * x] = (float)cos(a);
-        e[2 * x + 1] = (float)sin(a);
+        qdt::round_unit_complex(cos(a), sin(a), e[2 * x], e[2 * x + 1]);
       }
       float *d = nullptr;
       QD_RC(upload(&d, e.data(), e.size()));
@@ -1485,8 +1487,8 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     // fused theta step: CPU-propagator order, three pieces, both matrices in LDS
     size_t flds = 0;
     if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->variant == QD_VARIANT_CPU && h->nsplit == 3 &&
-        h->tmx_ks >= 1 && h->tmx_ks <= 2 && theta_fused_kernel(h->tmx_ks, &flds)) {
-      const void *fn = theta_fused_kernel(h->tmx_ks, &flds);
+        h->tmx_ks >= 1 && h->tmx_ks <= 2 && theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds)) {
+      const void *fn = theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds);
       QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
       int nbk = 0;
       QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, TMX_THREADS, flds));
@@ -1494,6 +1496,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
         h->fuse_theta = true;
         h->tfu_lds = flds;
         h->tfu_tpw = h->tmx_tpw;
+        h->tfu_nth = TMX_THREADS;
         QD_RC(build_tfu_items(h));
       }
     }
@@ -1996,6 +1999,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[9] = h->rp_mx ? std::max(h->mxs[0].gb, h->mxs[1].gb) : 0;
   vals[10] = h->fuse_theta ? 1 : 0;
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
+  vals[13] = h->fuse_theta ? h->tfu_nth : 0;
   vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
   return QD_OK;
 }
@@ -2325,6 +2329,7 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     h->mx_chunk = value;
     return QD_OK;
   }
+
   return fail(QD_ERR_ARG, "unknown selection");
 }
 

@@ -209,12 +209,18 @@ const void *theta_mx_kernel(int ks, int pro, int epi) {
 }
 
 // fused FBR -> DVR -> FBR kernel for KS k-steps (both matrices in LDS)
-const void *theta_fused_kernel(int ks, size_t *lds) {
-  switch (ks) {
-    case 1: *lds = theta_fused_lds<1>(); return reinterpret_cast<const void *>(&k_theta_fused<1>);
-    case 2: *lds = theta_fused_lds<2>(); return reinterpret_cast<const void *>(&k_theta_fused<2>);
-    default: return nullptr;
+const void *theta_fused_kernel(int ks, int nth, size_t *lds) {
+#define QD_TFU(KS_, NTH_)                                                \
+  if (ks == KS_ && nth == NTH_) {                                        \
+    *lds = theta_fused_lds<KS_, NTH_>();                                 \
+    return reinterpret_cast<const void *>(&k_theta_fused<KS_, NTH_>);    \
   }
+  // (768 threads, three waves per SIMD, fits the LDS but not the 168-register
+  // budget: it spills and measured 3.35 ms against 2.74 ms on the batched step)
+  QD_TFU(1, 512)
+  QD_TFU(2, 512)
+#undef QD_TFU
+  return nullptr;
 }
 #endif  // theta part
 

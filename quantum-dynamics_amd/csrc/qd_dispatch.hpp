@@ -37,7 +37,8 @@ const void *rpass_mx_kernel_small(int n, int nl, int gb, bool two, size_t *lds);
 // matrix-stationary mixed theta GEMM for ks k-steps, V-phase prologue pro, dump epilogue epi
 // (epi 2: FBR -> DVR times conj(exp(-i V dt)), the DVR state from the fused step's FBR state)
 const void *theta_mx_kernel(int ks, int pro, int epi);
-// fused FBR -> DVR -> FBR theta kernel (k_theta_fused) for ks k-steps; *lds = dynamic LDS bytes
-const void *theta_fused_kernel(int ks, size_t *lds);
+// fused FBR -> DVR -> FBR theta kernel (k_theta_fused) for ks k-steps and nth
+// threads per work group (512); *lds = dynamic LDS bytes
+const void *theta_fused_kernel(int ks, int nth, size_t *lds);
 
 }  // namespace qd

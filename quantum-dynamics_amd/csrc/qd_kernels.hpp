@@ -1152,8 +1152,8 @@ struct ThetaFusedArgs {
   const int4 *items;           // per work group: first block of its group, blocks, tile begin, tile end
 };
 
-template <int KS>
-__global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
+template <int KS, int NTH>
+__global__ __launch_bounds__(NTH) void k_theta_fused(ThetaFusedArgs P) {
   extern __shared__ __align__(16) unsigned char tm_lds[];
   constexpr int KP = KS * 32, NT = 2 * KS, NP = NT * 16, LD = KP + 8, XS = KP + 1;
   __bf16 *Mb = reinterpret_cast<__bf16 *>(tm_lds);  // [3][NP][LD], row n = DVR column, k = FBR column
@@ -1282,30 +1282,32 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    bf8v fre[KS][3], fim[KS][3];
-#pragma unroll
-    for (int s2 = 0; s2 < KS; s2++) {
-      float re[8], im[8];
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const float2 x = Xw[rl * XS + s2 * 32 + kg * 8 + e];
-        re[e] = x.x;
-        im[e] = x.y;
-      }
-      split3(re, fre[s2]);
-      split3(im, fim[s2]);
-    }
-    // reads done before the next tile's writes
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // forward GEMM: FBR[m] = sum_n Wf[n][m] DVR[n]
+    // forward GEMM: FBR[m] = sum_n Wf[n][m] DVR[n]; each k-step's B operand
+    // is split from the LDS tile just before its products
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int b = 0; b < NT; b++) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s2 = 0; s2 < KS; s2++)
+    for (int s2 = 0; s2 < KS; s2++) {
+      bf8v fre[3], fim[3];
+      {
+        float re[8], im[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float2 x = Xw[rl * XS + s2 * 32 + kg * 8 + e];
+          re[e] = x.x;
+          im[e] = x.y;
+        }
+        split3(re, fre);
+        split3(im, fim);
+      }
+      if (s2 + 1 == KS) {
+        // reads done before the next tile's writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) {
         bf8v a[3];
@@ -1315,10 +1317,11 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
         for (int d = 2; d >= 0; d--)
 #pragma unroll
           for (int p = 0; p <= d; p++) {
-            acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[s2][d - p], acc[0][nt], 0, 0, 0);
-            acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[s2][d - p], acc[1][nt], 0, 0, 0);
+            acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[d - p], acc[0][nt], 0, 0, 0);
+            acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[d - p], acc[1][nt], 0, 0, 0);
           }
       }
+    }
 #pragma unroll
     for (int nt = 0; nt < NT; nt++)
 #pragma unroll
@@ -1331,9 +1334,9 @@ __global__ __launch_bounds__(512) void k_theta_fused(ThetaFusedArgs P) {
     rL = rLn;
   }
 }
-template <int KS>
+template <int KS, int NTH>
 constexpr size_t theta_fused_lds() {
-  return (size_t)2 * 3 * (2 * KS * 16) * (KS * 32 + 8) * 2 + (size_t)(TMX_THREADS / 64) * 16 * (KS * 32 + 1) * 8;
+  return (size_t)2 * 3 * (2 * KS * 16) * (KS * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * (KS * 32 + 1) * 8;
 }
 
 template <int KS, int NT>

@@ -193,6 +193,40 @@ inline void coriolis_eigen(int J, int j, std::vector<double> &U, std::vector<int
   }
 }
 
+// FP32 pair (fc, fs) for the complex number c + i s of modulus rho, chosen
+// among the floats within 4 ulp of c and of s so that fc^2 + fs^2 is as close
+// to rho^2 as they allow (ties: the pair nearest the value).  A factor that
+// multiplies the same points every step then scales their norm by 1 + O(1e-10)
+// per application instead of by the 1 + O(1e-8) of a plain rounding, while its
+// phase is off by at most a few FP32 ulps.
+inline void round_unit_complex(double c, double s, float &fc, float &fs) {
+  const double rho2 = c * c + s * s;
+  const float c0 = (float)c, s0 = (float)s;
+  float cands_c[9], cands_s[9];
+  cands_c[4] = c0;
+  cands_s[4] = s0;
+  for (int k = 1; k <= 4; k++) {
+    cands_c[4 + k] = std::nextafter(cands_c[3 + k], INFINITY);
+    cands_c[4 - k] = std::nextafter(cands_c[5 - k], -INFINITY);
+    cands_s[4 + k] = std::nextafter(cands_s[3 + k], INFINITY);
+    cands_s[4 - k] = std::nextafter(cands_s[5 - k], -INFINITY);
+  }
+  double best = INFINITY, bdist = INFINITY;
+  fc = c0;
+  fs = s0;
+  for (float a : cands_c)
+    for (float b : cands_s) {
+      const double err = std::fabs((double)a * a + (double)b * b - rho2);
+      const double dist = std::fabs((double)a - c) + std::fabs((double)b - s);
+      if (err < best || (err == best && dist < bdist)) {
+        best = err;
+        bdist = dist;
+        fc = a;
+        fs = b;
+      }
+    }
+}
+
 // FP32 copy of an orthogonal g x g matrix U (row K, column m), rounded so that
 // every column keeps unit length to second order.  The mixing applies U^T and
 // then U to the same point every step; with plain rounding, U_f = U + E, the
