@@ -258,6 +258,10 @@ struct qd_handle {
   double2 *stage = nullptr;
   double2 *bpsi = nullptr, *bdpsi = nullptr, *fai = nullptr, *dfai = nullptr;
   double2 *ftau = nullptr;  // tau[t][e] of the current time block (k_flux_tau)
+  // partial P(E) sums of each 128-value x range, left by every fold
+  // (k_flux_accumulate): used where every block starts on such a range
+  double *crp_part = nullptr;
+  bool part_ok = false, part_valid = true;
   double *energies = nullptr, *eta_sq = nullptr, *crp = nullptr, *partial = nullptr;
   int npartial = 256;
   // timing
@@ -609,12 +613,13 @@ int flush_flux(qd_handle *h) {
   const long long nxb = (h->nx + FX_X - 1) / FX_X, ney = (h->ne + FX_E - 1) / FX_E;
   dim3 grid((unsigned)(((nxb + 7) / 8) * 8 * ney));
   hipLaunchKernelGGL(k_flux_accumulate, grid, dim3(256), 0, h->stream, h->bpsi, h->bdpsi, h->fai, h->dfai, h->ftau,
-                     (int)h->nx, h->ne, h->slot);
+                     (int)h->nx, h->ne, h->slot, h->part_ok ? h->crp_part : nullptr, h->dcol_w, h->n1);
   hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, h->stream, h->d_acc_steps, h->slot);
   QD_HIP(hipGetLastError());
   QD_RC(mark(h, -1));  // closes the accumulation's interval
   h->acc_steps += h->slot;
   h->slot = 0;
+  h->part_valid = true;
   return QD_OK;
 }
 
@@ -622,8 +627,12 @@ int compute_crp(qd_handle *h) {
   QD_RC(flush_flux(h));
   if (h->profile) QD_RC(drain_marks(h));
   const double sign = h->variant == QD_VARIANT_CUDA ? -1.0 : 1.0;
-  hipLaunchKernelGGL(k_crp, dim3(h->ne, h->nb), dim3(256), 0, h->stream, h->fai, h->dfai, h->dcol_w, h->eta_sq,
-                     h->d_blocks, h->crp, h->n1, (int)h->nx, h->ne, h->dr1 / h->mu2, sign);
+  if (h->part_ok && h->part_valid)
+    hipLaunchKernelGGL(k_crp_parts, dim3(h->ne, h->nb), dim3(64), 0, h->stream, h->crp_part, h->eta_sq, h->d_blocks,
+                       h->crp, h->n1, h->ne, h->dr1 / h->mu2, sign);
+  else
+    hipLaunchKernelGGL(k_crp, dim3(h->ne, h->nb), dim3(256), 0, h->stream, h->fai, h->dfai, h->dcol_w, h->eta_sq,
+                       h->d_blocks, h->crp, h->n1, (int)h->nx, h->ne, h->dr1 / h->mu2, sign);
   QD_HIP(hipGetLastError());
   h->host_crp.resize((size_t)h->nb * h->ne);
   QD_HIP(hipMemcpyAsync(h->host_crp.data(), h->crp, sizeof(double) * h->host_crp.size(), hipMemcpyDeviceToHost,
@@ -1526,6 +1535,15 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMalloc((void **)&h->dfai, sizeof(double2) * h->nx * h->ne));
     QD_HIP(hipMemsetAsync(h->fai, 0, sizeof(double2) * h->nx * h->ne, h->stream));
     QD_HIP(hipMemsetAsync(h->dfai, 0, sizeof(double2) * h->nx * h->ne, h->stream));
+    // P(E) from the folds' partial sums where every block starts on a 128-value x range
+    h->part_ok = true;
+    for (const BlockDesc &d : h->hblocks)
+      if (((long long)n1 * d.dcol0) % FX_X) h->part_ok = false;
+    if (h->part_ok) {
+      const size_t nparts = (h->nx + FX_X - 1) / FX_X * (size_t)h->ne;
+      QD_HIP(hipMalloc((void **)&h->crp_part, sizeof(double) * nparts));
+      QD_HIP(hipMemsetAsync(h->crp_part, 0, sizeof(double) * nparts, h->stream));
+    }
   }
   QD_HIP(hipMalloc((void **)&h->partial, sizeof(double) * 4 * h->npartial));  // 4 diagnostic slots
   QD_HIP(hipStreamSynchronize(h->stream));
@@ -1700,7 +1718,7 @@ void release(qd_handle *h) {
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
-                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab};
+                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab, h->crp_part};
   drop_graph(h);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -2287,6 +2305,7 @@ int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes) {
   }
   h->steps = (int32_t)c.steps;
   h->acc_steps = c.acc_steps;
+  h->part_valid = false;  // the loaded accumulators' P(E) partials are rebuilt by the next fold
   h->slot = (int)c.slot;
   h->state_fbr = c.state_fbr != 0;
   h->dvr_valid = !h->state_fbr;

@@ -1384,16 +1384,18 @@ __global__ void k_flux_extract(const typename cx_of<T>::type *__restrict__ psi, 
 // tau[t, e] = dt exp(i E_e (n0 + t + 1) dt)  (the reference's recursion
 // timeEvol.C:799-817 evaluated directly, as evolutionCUDA2.cu:143-145 does).
 // k_flux_tau writes the block's tau table once; k_flux_accumulate stages it
-// and the block's surface values through LDS.  Work group: 128 x-values x 32
-// energies; each thread holds 2 x-values (xl, xl + 64) x 8 energies, so every
+// and the block's surface values through LDS.  Work group: 128 x-values x 16
+// energies; each thread holds 2 x-values (xl, xl + 64) x 4 energies, so every
 // tau value read from LDS feeds two complex products per array; each complex
 // multiply-add is four FMAs.  Per (x, e) the sum runs over t in increasing
 // order.  The work groups of one x range (its energy blocks) get linear ids
 // 8 apart, so they run on one XCD and read the surface values from its L2.
+// (8 energies per thread: 194 registers, two waves per SIMD, and a 20-step
+// fold of the batched workload took 12.6 ms; 4 per thread: 9.6 ms.)
 // ---------------------------------------------------------------------------
 constexpr int FX_X = 128;
-constexpr int FX_E = 32;
-constexpr int FX_EPT = 8;
+constexpr int FX_E = 16;
+constexpr int FX_EPT = 4;
 constexpr int FX_TC = 8;
 
 // counter += add (one thread): advances device-side step counts between
@@ -1415,10 +1417,17 @@ static __global__ __launch_bounds__(256) void k_flux_tau(double2 *__restrict__ t
   tau[i] = make_double2(c * dt, s * dt);
 }
 
+// With part != nullptr it also leaves, for its 128 x-values and each of its
+// energies, the partial sum  sum_x w_x Im(conj(fai) dfai)  of the updated
+// accumulators in part[xb][e] (w_x = dcol_w[x / n1]), so that P(E) after the
+// fold needs these partials only (k_crp_parts) instead of another pass over
+// fai and dfai.
 static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *__restrict__ bpsi,
                                                          const double2 *__restrict__ bdpsi,
                                                          double2 *__restrict__ fai, double2 *__restrict__ dfai,
-                                                         const double2 *__restrict__ tau, int nx, int ne, int nt) {
+                                                         const double2 *__restrict__ tau, int nx, int ne, int nt,
+                                                         double *__restrict__ part, const double *__restrict__ dcol_w,
+                                                         int n1) {
   __shared__ double2 sb[FX_TC][FX_X];
   __shared__ double2 sdb[FX_TC][FX_X];
   __shared__ double2 stau[FX_TC][FX_E];
@@ -1475,21 +1484,55 @@ static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *_
     }
     __syncthreads();
   }
+  double pim[FX_EPT];
+#pragma unroll
+  for (int q = 0; q < FX_EPT; q++) pim[q] = 0.0;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int x = xb * FX_X + xl + 64 * h;
     if (x < nx) {
+      const double w = part ? dcol_w[x / n1] : 0.0;
 #pragma unroll
       for (int q = 0; q < FX_EPT; q++) {
         const int e = e0 + eg * FX_EPT + q;
         if (e < ne) {
           const size_t o = (size_t)e * nx + x;
-          fai[o] = cadd(fai[o], af[h][q]);
-          dfai[o] = cadd(dfai[o], adf[h][q]);
+          const double2 a = cadd(fai[o], af[h][q]), b = cadd(dfai[o], adf[h][q]);
+          fai[o] = a;
+          dfai[o] = b;
+          pim[q] = fma(w, a.x * b.y - a.y * b.x, pim[q]);  // w Im(conj(a) b)
         }
       }
     }
   }
+  if (part) {
+    // one wave per energy group: sum its 64 lanes (fixed order: a tree over lanes)
+#pragma unroll
+    for (int q = 0; q < FX_EPT; q++) {
+      double v = pim[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      const int e = e0 + eg * FX_EPT + q;
+      if (xl == 0 && e < ne) part[(size_t)xb * ne + e] = v;
+    }
+  }
+}
+
+// P(E) of block b from the partials of k_flux_accumulate: the sum over the
+// block's 128-value x ranges (blocks start on such a range), times the same
+// factors as k_crp.  Grid (ne, nblocks); the sum runs in a fixed order.
+static __global__ __launch_bounds__(64) void k_crp_parts(const double *__restrict__ part,
+                                                  const double *__restrict__ eta_sq,
+                                                  const BlockDesc *__restrict__ blocks, double *__restrict__ crp,
+                                                  int n1, int ne, double dr1_mu2, double sign) {
+  const int e = blockIdx.x;
+  const BlockDesc bd = blocks[blockIdx.y];
+  const int t0 = n1 * bd.dcol0 / FX_X, t1 = (n1 * (bd.dcol0 + bd.nd) + FX_X - 1) / FX_X;
+  double s = 0.0;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) s += part[(size_t)t * ne + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) crp[(size_t)blockIdx.y * ne + e] = sign * s / eta_sq[(size_t)blockIdx.y * ne + e] * dr1_mu2;
 }
 
 // P(E) of block b: Im(sum_k w_k sum_i conj(fai) dfai) / eta^2 * dr1/mu2
