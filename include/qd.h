@@ -239,12 +239,16 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
  * [9] its unit-size bucket, [10] fused theta step in use, [11] its 16-row
  * tiles per work group, [12] that mixing R pass runs two work groups per CU,
- * [13] threads per work group of the fused theta step.  vals holds 14 entries. */
+ * [13] threads per work group of the fused theta step, [14] the split mixing
+ * R pass (K -> l mixing, the register FFT pass without mixing, l -> K mixing,
+ * three launches) is in use.  vals holds 15 entries. */
 int qd_get_tuning(qd_handle *h, int32_t *vals);
 /* test hook: force one kernel choice the autotuner makes.  what 0: the
  * prefetching FP32 mixing R pass with one (value 0) or two (value 1) work
  * groups per CU; QD_ERR_ARG when the handle does not have that kernel.
- * what 1: its consecutive bundles per turn of a work group (1..64, default 8). */
+ * what 1: its consecutive bundles per turn of a work group (1..64, default 8).
+ * what 2: the split mixing R pass on (value 1) or off (value 0); QD_ERR_ARG
+ * when the handle does not have it, or has no other R pass for its groups. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

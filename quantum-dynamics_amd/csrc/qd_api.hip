@@ -30,6 +30,7 @@
 #include "qd_fft.hpp"
 #include "qd_rfft.hpp"
 #include "qd_kernels.hpp"
+#include "qd_mix.hpp"
 #include "qd_dispatch.hpp"
 #include "qd_pack.hpp"
 #include "qd_records.hpp"
@@ -228,6 +229,12 @@ struct qd_handle {
   int mx_chunk = MX_CHUNK;               // consecutive bundles per turn of a work group
   float *mx_umat = nullptr;              // unit matrices of both classes
   long long *line_off = nullptr;
+  // split mixing R pass: K -> l mixing into a scratch array, the register FFT
+  // pass without mixing there, l -> K mixing back (any group size)
+  bool rp_split = false, sp_ok = false;
+  int sp_nunits = 0, sp_wpu = 1, sp_lw = 0;  // units, waves per unit, lines per FFT work group
+  long long sp_slots = 1;
+  void *sp_tmp = nullptr;  // scratch of nlp points when psi is smaller
   size_t tmx_lds = 0;
   // fused theta step (k_theta_fused): between steps the state is the FBR array
   // holding theta-forward(V psi); psi itself is rebuilt from it when read
@@ -381,6 +388,27 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.line_off = h->line_off;
   a.line_lam = h->line_lam;
   a.nbundle = h->nbundle;
+  if (h->rp_split && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
+    C *tmp = (C *)(h->sp_tmp ? h->sp_tmp : (h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32));
+    const unsigned mblk = (unsigned)(((long long)h->sp_nunits * h->sp_wpu + 3) / 4);
+    hipLaunchKernelGGL(k_mix_lines<T>, dim3(mblk), dim3(256), 0, h->stream, (const C *)data, tmp, h->units,
+                       h->unit_cols, (const T *)h->umat, h->sp_nunits, h->n1, h->n2, h->sp_wpu, 1);
+    QD_HIP(hipGetLastError());
+    RPassArgs<T> b = a;
+    b.data = tmp;
+    b.flags = flags & ~FFT_MIX;
+    size_t lds = 0;
+    const void *fn = rpass_rf_kernel<T>(h->n1, false, h->sp_lw, &lds);
+    void *args[] = {&b};
+    const unsigned nblk =
+        (unsigned)std::min(h->sp_slots, ((long long)h->nfcol * h->n2 + h->sp_lw - 1) / h->sp_lw);
+    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->sp_lw * rf_n1(h->n1)), args, lds, h->stream));
+    QD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mix_lines<T>, dim3(mblk), dim3(256), 0, h->stream, (const C *)tmp, data, h->units,
+                       h->unit_cols, (const T *)h->umat, h->sp_nunits, h->n1, h->n2, h->sp_wpu, 0);
+    QD_HIP(hipGetLastError());
+    return QD_OK;
+  }
   if constexpr (sizeof(T) == 4) {
     if (h->rp_mx && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
       RMixArgs m;
@@ -1080,15 +1108,34 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   int gmax = 1;
   for (const Group &g : S.groups) gmax = std::max(gmax, (int)g.cols.size());
   // the FP32 mixing R pass with prefetch takes mixing groups of up to 48 lines
-  // on 64-, 128- and 256-point R grids; the other R passes up to 32
+  // on 64-, 128- and 256-point R grids, the LDS-staged and Stockham R passes up
+  // to 32; the split R pass (mixing in passes of its own around the register
+  // FFT pass without mixing) any number, where that FFT kernel exists
   const bool mx_cand = h->prec == QD_PREC_MIXED && h->has_mix && (n1 == 64 || n1 == 128 || n1 == 256) && gmax <= 48;
-  if (gmax > RP_GMAX && !mx_cand)
-    return fail(QD_ERR_ARG, "mixing groups larger than 32 lines (J > 31) need mixed precision on a 64-, 128- or "
-                            "256-point R grid, and at most 48 lines (J <= 47)");
+  bool sp_cand = false;
+  if (h->has_mix && rf_length(n1)) {
+    const int lw = 256 / rf_n1(n1);
+    size_t lds = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, false, lw, &lds)
+                                             : rpass_rf_kernel<float>(n1, false, lw, &lds);
+    sp_cand = fn && lds <= 160 * 1024 && n2 % lw == 0;
+    h->sp_lw = lw;
+  }
+  if (gmax > RP_GMAX && !mx_cand && !sp_cand)
+    return fail(QD_ERR_ARG, "mixing groups larger than 32 lines (J > 31) need an R grid of a length with a "
+                            "register FFT kernel (" + std::to_string(n1) + " has none)");
   h->rp_legacy = gmax <= RP_GMAX;
   std::vector<int> bu0, ucols;
   std::vector<RUnit> units;
   const size_t tabsz = h->prec == QD_PREC_FP64 ? sizeof(double2) : sizeof(float4);
+  if (!h->rp_legacy && sp_cand) {
+    // the units alone (the split pass has no bundles)
+    for (const Group &g : S.groups) {
+      const int c0 = (int)ucols.size();
+      for (int c : g.cols) ucols.push_back(c);
+      for (int r = 0; r < n2; r++) units.push_back(RUnit{c0, (int)g.cols.size(), g.u0, r});
+    }
+  }
   if (h->rp_legacy) {
     // R pass: bundles of whole units; about 2048 elements per work group, at
     // least one mixing group, one line per thread slot.  Lengths with a
@@ -1247,6 +1294,20 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds));
     h->rl_slots = std::max(1, nb) * std::max(1, ncu);
   }
+  if (sp_cand && !units.empty()) {
+    size_t lds = 0;
+    const void *fn = h->prec == QD_PREC_FP64 ? rpass_rf_kernel<double>(n1, false, h->sp_lw, &lds)
+                                             : rpass_rf_kernel<float>(n1, false, h->sp_lw, &lds);
+    QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nb = 0;
+    QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, h->sp_lw * rf_n1(n1), lds));
+    h->sp_slots = std::max(1, nb) * std::max(1, ncu);
+    h->sp_nunits = (int)units.size();
+    h->sp_wpu = (n1 + 63) / 64;
+    h->sp_ok = true;
+    // the default where no single-launch mixing pass takes these groups
+    h->rp_split = !h->rp_legacy && !mx_cand;
+  }
   QD_RC(upload(&h->d_blocks, h->hblocks.data(), h->hblocks.size()));
   {
     std::vector<BlockDesc> byk = h->hblocks;
@@ -1297,6 +1358,9 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMalloc((void **)&h->psi32, sizeof(float2) * h->npt));
     QD_HIP(hipMalloc((void **)&h->lpsi32, sizeof(float2) * h->nlp));
   }
+  // the split R pass mixes into psi, which no step reads across the R pass;
+  // a DVR array smaller than the FBR one takes a scratch array instead
+  if (h->sp_ok && h->npt < h->nlp) QD_HIP(hipMalloc(&h->sp_tmp, elem * h->nlp));
   for (int b = 0; b < h->nb; b++) {
     const BlockDesc &d = h->hblocks[b];
     QD_RC(put_complex(h, h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32, h->n12 * d.dcol0,
@@ -1718,7 +1782,7 @@ void release(qd_handle *h) {
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
-                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab, h->crp_part};
+                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab, h->crp_part, h->sp_tmp};
   drop_graph(h);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -1934,32 +1998,38 @@ int autotune_handle(qd_handle *h) {
     return QD_OK;
   };
   // R pass: the prefetching mixing kernel (one or two work groups per CU),
-  // the register FFT with LDS-staged mixing, or the Stockham kernels,
-  // whichever of them this handle has
+  // the register FFT with LDS-staged mixing, the Stockham kernels or the split
+  // pass, whichever of them this handle has
   {
     const bool has_mx = h->rp_mx, has_mx2 = h->rp_mx && h->mxs[0].slots2 > 0, has_rf = h->rp_rf,
-               has_st = h->rp_legacy;
+               has_st = h->rp_legacy, has_sp = h->sp_ok;
     auto runR = [&] {
       return f64 ? launch_rpass<double>(h, full, (double2 *)lpsi) : launch_rpass<float>(h, full, (float2 *)lpsi);
     };
-    float best = 1e30f;
-    int pick = has_mx ? 0 : has_rf ? 1 : 2;
-    for (int c = 0; c < 4; c++) {
-      if ((c == 0 && !has_mx) || (c == 1 && !has_rf) || (c == 2 && !has_st) || (c == 3 && !has_mx2)) continue;
-      if (c == 2 && !has_mx && !has_rf) continue;  // the only choice
-      h->rp_mx = c == 0 || c == 3;
-      h->mx_two = c == 3;
-      h->rp_rf = c == 1;
-      float ms = 0;
-      QD_RC(time_launches(h, runR, reps, &ms));
-      if (ms < best) {
-        best = ms;
-        pick = c;
+    const bool cand[5] = {has_mx, has_rf, has_st, has_mx2, has_sp};
+    int ncand = 0;
+    for (bool c : cand) ncand += c;
+    if (ncand > 1) {
+      float best = 1e30f;
+      int pick = -1;
+      for (int c = 0; c < 5; c++) {
+        if (!cand[c]) continue;
+        h->rp_mx = c == 0 || c == 3;
+        h->mx_two = c == 3;
+        h->rp_rf = c == 1;
+        h->rp_split = c == 4;
+        float ms = 0;
+        QD_RC(time_launches(h, runR, reps, &ms));
+        if (ms < best) {
+          best = ms;
+          pick = c;
+        }
       }
+      h->rp_mx = pick == 0 || pick == 3;
+      h->mx_two = pick == 3;
+      h->rp_rf = pick == 1;
+      h->rp_split = pick == 4;
     }
-    h->rp_mx = pick == 0 || pick == 3;
-    h->mx_two = pick == 3;
-    h->rp_rf = pick == 1;
   }
   QD_RC(pick_pass(&h->rl_rf, false));
   h->tuned = true;
@@ -2019,6 +2089,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
   vals[13] = h->fuse_theta ? h->tfu_nth : 0;
   vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
+  vals[14] = h->rp_split ? 1 : 0;
   return QD_OK;
 }
 
@@ -2339,7 +2410,19 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     drop_graph(h);
     h->rp_mx = true;
     h->rp_rf = false;
+    h->rp_split = false;
     h->mx_two = value == 1;
+    return QD_OK;
+  }
+  if (what == 2) {
+    if (value != 0 && value != 1) return fail(QD_ERR_ARG, "value must be 0 or 1");
+    if (value == 1 && !h->sp_ok) return fail(QD_ERR_ARG, "this handle has no split mixing R pass");
+    if (value == 0 && !h->rp_legacy && !(h->mxs[0].nbundle || h->mxs[1].nbundle))
+      return fail(QD_ERR_ARG, "the split R pass is the only one these mixing groups fit");
+    drop_graph(h);
+    h->rp_split = value == 1;
+    if (value == 1) h->rp_mx = h->rp_rf = false;
+    else if (!h->rp_legacy) h->rp_mx = true;
     return QD_OK;
   }
   if (what == 1) {

@@ -194,12 +194,13 @@ class Propagator:
 
     def tuning(self) -> dict:
         """Kernel configuration in use (qd_get_tuning)."""
-        v = (C.c_int32 * 14)()
+        v = (C.c_int32 * 15)()
         _abi.check(self.lib.qd_get_tuning(self.h, v))
         keys = ("autotuned", "theta_ksteps", "theta_tiles_per_group", "rpass_register_fft",
                 "rline_register_fft", "rpass_lines_per_group", "mixing_bundle_lines", "rline_lines_per_group",
                 "rpass_prefetch_mix_bundle_lines", "rpass_prefetch_mix_group_bucket", "fused_theta",
-                "fused_theta_tiles_per_group", "rpass_prefetch_mix_two_groups", "fused_theta_threads")
+                "fused_theta_tiles_per_group", "rpass_prefetch_mix_two_groups", "fused_theta_threads",
+                "rpass_split_mix")
         return dict(zip(keys, list(v)))
 
     def timing(self, which: int) -> float:

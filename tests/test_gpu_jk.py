@@ -116,19 +116,26 @@ def test_jk_prefetch_mixing_mixed_within_1e4_of_fp64(qd):
     assert abs(n1 - n0) < 1e-5 * n0
 
 
-def test_jk_groups_above_32_lines_match_oracle(qd):
-    """J = 33: 34 Coriolis-coupled K blocks per (J, j >= 33), bundled 48 lines
-    at a time by the prefetching mixing R pass (mixed precision only), against
-    the FP64 numpy oracle."""
+@pytest.mark.parametrize("path", ["mixed-prefetch48", "mixed-split", "fp64-split"])
+def test_jk_groups_above_32_lines_match_oracle(qd, path):
+    """J = 33: 34 Coriolis-coupled K blocks per (J, j >= 33), against the FP64
+    numpy oracle: mixed precision with the prefetching mixing R pass (48-line
+    bundles) or the split pass, and FP64, whose only R pass for groups above
+    32 lines is the split one."""
     import jk_oracle
 
     pb = qd.problem
-    base = pb.config(0, nR=64, nr=16, ntheta=8, steps=12, e0_ev=0.6)
+    base = pb.config(0, nR=64, nr=32, ntheta=8, steps=12, e0_ev=0.6)
     jk = pb.make_jk_problem(base, pb.H_H2, [33], jmax=35, R0=8.0, delta=0.3, e0_ev=0.6)
-    print("case: jk J=33 groups of 34 lines, mixed vs oracle", flush=True)
+    print("case: jk J=33 groups of 34 lines vs oracle,", path, flush=True)
     psi_o, PJ, _ = jk_oracle.propagate(jk, 12)
-    with qd.propagator.JKPropagator(jk, precision="mixed") as prop:
-        assert prop.tuning()["rpass_prefetch_mix_bundle_lines"] == 48
+    prec = path.split("-")[0]
+    with qd.propagator.JKPropagator(jk, precision=prec) as prop:
+        if path == "mixed-prefetch48":
+            assert prop.tuning()["rpass_prefetch_mix_bundle_lines"] == 48
+        elif path == "mixed-split":
+            prop.debug_select(2, 1)
+        assert prop.tuning()["rpass_split_mix"] == (0 if path == "mixed-prefetch48" else 1)
         prop.propagate(12)
         psi_g = prop.psi().view(np.complex128)
-    assert rel(psi_g, psi_o) < 2e-5
+    assert rel(psi_g, psi_o) < (1e-10 if prec == "fp64" else 2e-5)

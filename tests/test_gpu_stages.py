@@ -118,9 +118,13 @@ def test_fused_passes_with_phases(qd, small, prec):
 # (R points, r points, J values, j_max): LDS-staged mixing (32 points), the
 # prefetching FP32 kernel with 32-line bundles (64 points, groups up to 10
 # lines; "two": its variant for two work groups per CU, on 256 points with
-# groups of up to 21 lines) and with 48-line bundles (groups of 34 lines)
+# groups of up to 21 lines) and with 48-line bundles (groups of 34 lines), and
+# the split pass (mixing in launches of its own around the FFT pass) on 64-,
+# 128- (groups of 41 lines) and 256-point R grids
 MIX_CASES = {"staged": (32, 24, [0, 2, 3], 5), "prefetch32": (64, 16, [0, 3, 9], 11),
-             "prefetch32two": (256, 16, [0, 4, 20], 22), "prefetch48": (64, 16, [33], 35)}
+             "prefetch32two": (256, 16, [0, 4, 20], 22), "prefetch48": (64, 16, [33], 35),
+             "split64": (64, 32, [0, 3, 9], 11), "split128": (128, 16, [40], 42),
+             "split256": (256, 16, [0, 4, 20], 22)}
 
 
 @pytest.mark.parametrize("case", sorted(MIX_CASES))
@@ -129,8 +133,9 @@ def test_r_pass_mixing_jk(qd, prec, case):
     """J > 0: the R pass mixes the K blocks of each (J, j) into the l basis,
     applies D_R/2 with l(l+1), transforms, and mixes back."""
     nR, nr, Js, jmax = MIX_CASES[case]
-    if prec == "fp64" and max(Js) > 31:
-        pytest.skip("mixing groups above 32 lines run in mixed precision only")
+    if prec == "fp64" and case == "prefetch48":
+        pytest.skip("FP64 groups above 32 lines take the split pass, whose 32-line FFT batches "
+                    "need a multiple of 32 r points on a 64-point R grid (this case has 16)")
     pb = qd.problem
     base = pb.config(0, nR=nR, nr=nr, ntheta=6, steps=2)
     jk = pb.make_jk_problem(base, pb.H_H2, Js, jmax=jmax)
@@ -148,11 +153,15 @@ def test_r_pass_mixing_jk(qd, prec, case):
             assert prop.tuning()["rpass_prefetch_mix_two_groups"] == two
             if case == "prefetch32":
                 prop.debug_select(1, 3)  # three consecutive bundles per turn of a work group
+        if case.startswith("split"):
+            prop.debug_select(2, 1)
         prop.set_fbr(as_f64(y))
         prop.debug_stage(1)
         got = prop.get_fbr().view(np.complex128)
         uses_prefetch = prop.tuning()["rpass_prefetch_mix_bundle_lines"] > 0
+        uses_split = prop.tuning()["rpass_split_mix"] == 1
     assert uses_prefetch == (prec == "mixed" and case.startswith("prefetch"))
+    assert uses_split == case.startswith("split")
     # numpy statement
     ref = y.copy()
     cols = {}
