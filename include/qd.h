@@ -121,11 +121,13 @@ typedef struct qd_options {
                              products) or 3 (six products); 0 = 3 */
   int32_t autotune;       /* 1: time the candidate kernel configurations on the
                              handle's own arrays at creation and keep the fastest */
-  int32_t fuse_theta;     /* mixed precision, CPU-propagator order, angular basis of at
-                             most 64: 0 = fuse the theta-backward GEMM of each step with
-                             the dump, the flux rows, the V phase and the next
-                             theta-forward GEMM (the state between steps is then the FBR
-                             array; psi is rebuilt from it when read); -1 = never */
+  int32_t fuse_theta;     /* mixed precision, angular basis of at most 64: 0 = fuse the
+                             theta-backward GEMM of each step with the dump, the flux
+                             rows, the V phase and the next theta-forward GEMM (the
+                             state between steps is then the FBR array; psi is rebuilt
+                             from it when read, or, in the CUDA propagator's order,
+                             whose FBR state holds the dumped psi, written by the last
+                             step before it is read); -1 = never */
   int32_t diagnostics_every; /* >0: every N-th step also evaluates the reference's
                              per-step diagnostics -- module, e_kin, e_rot, e_pot
                              (timeEvol.C:619-647,671) -- at the same points of the
@@ -179,7 +181,8 @@ int qd_destroy(qd_handle *h);
 int qd_set_psi(qd_handle *h, const double *psi_in);
 
 /* Checkpoint of the whole propagation state -- the wavefunction as the handle
- * holds it (psi, or the FBR array of the fused theta step), the energy-resolved
+ * holds it (psi, or the FBR array of the fused theta step, with psi beside it
+ * in the CUDA propagator's order), the energy-resolved
  * flux accumulators phi(E) and dphi(E) with the surface slices of the open time
  * block, the step counts and the pending first half step -- into a caller
  * buffer of qd_checkpoint_size bytes.  Loading it into a handle created from

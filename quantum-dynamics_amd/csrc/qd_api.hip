@@ -566,7 +566,9 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
 }
 
 // rows = false leaves psi untouched (no flux rows), for the autotuner
-int launch_theta_fused(qd_handle *h, bool rows = true) {
+// rows: the flux surface's psi rows; all: every row of psi (the CUDA
+// propagator's order, whose pre-dump psi the FBR state cannot give back)
+int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   ThetaFusedArgs F;
   F.L = h->lpsi32;
   F.psi = h->psi32;
@@ -580,8 +582,9 @@ int launch_theta_fused(qd_handle *h, bool rows = true) {
   F.d1f = h->has_dump ? h->d1f : nullptr;
   F.d2f = h->has_dump ? h->d2f : nullptr;
   F.n1 = h->n1;
-  F.r_lo = h->do_crp && rows ? h->surf - h->stencil.h : 1;
-  F.r_hi = h->do_crp && rows ? h->surf + h->stencil.h : 0;
+  F.r_lo = all ? 0 : h->do_crp && rows ? h->surf - h->stencil.h : 1;
+  F.r_hi = all ? h->n2 - 1 : h->do_crp && rows ? h->surf + h->stencil.h : 0;
+  F.pre_dump = h->variant == QD_VARIANT_CUDA ? 1 : 0;
   F.items = h->d_tfu_items;
   size_t lds = 0;
   const void *fn = theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
@@ -620,14 +623,20 @@ int build_tfu_items(qd_handle *h) {
 // Into the fused state: the FBR array becomes theta-forward(V psi).
 int enter_fbr_state(qd_handle *h) {
   if (!h->fuse_theta || h->state_fbr) return QD_OK;
-  QD_RC(launch_theta(h, true, 1, 0));
+  // the CUDA propagator's step starts with the dump (psi itself is not changed)
+  QD_RC(launch_theta(h, true, h->variant == QD_VARIANT_CUDA && h->has_dump ? 2 : 1, 0));
   h->state_fbr = true;
   return QD_OK;
 }
 // psi from the fused state: conj(exp(-i V dt)) times theta-backward of the
-// FBR array (the forward and backward transforms are exact inverses).
+// FBR array (the forward and backward transforms are exact inverses).  In the
+// CUDA propagator's order the FBR state holds the dumped psi, so there the
+// fused step that ends a call, precedes a diagnostics step or lands on a
+// callback point writes all of psi instead (step_impl), and psi is always
+// valid where it is read.
 int materialize_psi(qd_handle *h) {
   if (!h->fuse_theta || !h->state_fbr || h->dvr_valid) return QD_OK;
+  if (h->variant == QD_VARIANT_CUDA) return fail(QD_ERR_ARG, "internal: psi of the CUDA-order fused state is not held");
   QD_RC(launch_theta(h, false, 0, 2));
   h->dvr_valid = true;
   return QD_OK;
@@ -780,8 +789,10 @@ int finish_diag(qd_handle *h, int32_t steps_after) {
   return QD_OK;
 }
 
+// psi_out: psi is read after this step (the call ends, a diagnostics step or
+// a callback follows); the CUDA-order fused step then writes all of it.
 template <typename T>
-int step_impl(qd_handle *h, bool diag = false) {
+int step_impl(qd_handle *h, bool diag = false, bool psi_out = false) {
   typedef typename cx_of<T>::type C;
   C *psi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->psi64 : (void *)h->psi32);
   C *lpsi = (C *)(h->prec == QD_PREC_FP64 ? (void *)h->lpsi64 : (void *)h->lpsi32);
@@ -794,16 +805,18 @@ int step_impl(qd_handle *h, bool diag = false) {
     h->state_fbr = false;
     QD_RC(plain_step<T>(h, true));
   } else if (h->fuse_theta) {
-    // state on entry: the FBR array holds theta-forward(V psi_n); on exit the
-    // same for psi_{n+1}.  The psi rows of the flux surface are written by the
-    // fused kernel, after the dump, for k_flux_extract below.
+    // state on entry: the FBR array holds theta-forward(V psi_n) (CUDA order:
+    // of V dump psi_n); on exit the same for psi_{n+1}.  The psi rows of the
+    // flux surface are written by the fused kernel, after the dump (CUDA
+    // order: before it), for k_flux_extract below.
+    const bool all = cuda && psi_out;
     QD_RC(mark(h, TK_FFTR1));
     QD_RC(launch_rpass<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
     QD_RC(mark(h, TK_FFTR2));
     QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
     QD_RC(mark(h, TK_THETA));
-    QD_RC(launch_theta_fused(h));
-    h->dvr_valid = false;
+    QD_RC(launch_theta_fused(h, true, all));
+    h->dvr_valid = all;
   } else if (h->pipe == QD_PIPE_FUSED) {
     // CUDA propagator: dump at the start of the step (evolutionCUDA2.cu:503),
     // fused with the potential phase; CPU propagator: dump after the step.
@@ -1557,9 +1570,9 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMemcpy(db, B.data(), B.size() * 2, hipMemcpyHostToDevice));
     h->Wf = df;
     h->Wb = db;
-    // fused theta step: CPU-propagator order, three pieces, both matrices in LDS
+    // fused theta step: either propagator order, three pieces, both matrices in LDS
     size_t flds = 0;
-    if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->variant == QD_VARIANT_CPU && h->nsplit == 3 &&
+    if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->nsplit == 3 &&
         h->tmx_ks >= 1 && h->tmx_ks <= 2 && theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds)) {
       const void *fn = theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds);
       QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
@@ -1801,8 +1814,10 @@ int capture_block(qd_handle *h) {
   const long long acc0 = h->acc_steps;
   QD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = QD_OK;
+  // the block's last step leaves psi valid (a CUDA-order fused handle)
   for (int k = 0; k < h->tblock && rc == QD_OK; k++)
-    rc = h->prec == QD_PREC_FP64 ? step_impl<double>(h) : step_impl<float>(h);
+    rc = h->prec == QD_PREC_FP64 ? step_impl<double>(h, false, k + 1 == h->tblock)
+                                 : step_impl<float>(h, false, k + 1 == h->tblock);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(h->stream, &g);
   h->slot = slot0;
@@ -1858,9 +1873,15 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
       h->steps += B;
       s += B;
       if (h->do_crp) h->acc_steps += B;
+      if (h->fuse_theta && h->variant == QD_VARIANT_CUDA) h->dvr_valid = true;  // its last step wrote psi
     } else {
-      if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h, diag));
-      else QD_RC(step_impl<float>(h, diag));
+      // psi is read after this step: the call ends, a diagnostics step or a
+      // callback follows
+      const bool psi_out = s + 1 == nsteps ||
+                           (h->diag_every > 0 && (h->steps + 2) % h->diag_every == 0) ||
+                           (h->crp_every > 0 && (h->cb || h->do_crp) && (h->steps + 1) % h->crp_every == 0);
+      if (h->prec == QD_PREC_FP64) QD_RC(step_impl<double>(h, diag, psi_out));
+      else QD_RC(step_impl<float>(h, diag, psi_out));
       h->steps++;
       s++;
     }
@@ -2286,9 +2307,13 @@ struct CkptHead {
   int64_t npt, nlp, nx, ne, tblock, prec, nb;      // shape of the handle it came from
   int64_t steps, acc_steps, slot, state_fbr, pending_prephase, bytes;
 };
+// A CUDA-order fused handle keeps psi beside its FBR state (the state holds the
+// dumped psi): its checkpoints carry both.
+bool ckpt_both(const qd_handle *h) { return h->fuse_theta && h->variant == QD_VARIANT_CUDA; }
 int64_t ckpt_bytes(const qd_handle *h) {
   const int64_t cs = h->prec == QD_PREC_FP64 ? 16 : 8;
   int64_t b = sizeof(CkptHead) + cs * (int64_t)std::max(h->npt, h->nlp);
+  if (ckpt_both(h)) b += cs * (int64_t)h->npt;
   if (h->do_crp) b += 2 * 16 * (int64_t)h->nx * h->ne + 2 * 16 * (int64_t)h->nx * h->tblock;
   return b;
 }
@@ -2323,7 +2348,8 @@ int qd_checkpoint_save(qd_handle *h, void *buf, int64_t bytes) {
   c.steps = h->steps;
   c.acc_steps = h->acc_steps;
   c.slot = h->slot;
-  c.state_fbr = h->state_fbr ? 1 : 0;
+  // 2: the FBR state and psi (a CUDA-order fused handle between calls)
+  c.state_fbr = h->state_fbr ? (ckpt_both(h) && h->dvr_valid ? 2 : 1) : 0;
   c.pending_prephase = h->pending_prephase ? 1 : 0;
   char *p = static_cast<char *>(buf);
   memcpy(p, &c, sizeof c);
@@ -2336,6 +2362,11 @@ int qd_checkpoint_save(qd_handle *h, void *buf, int64_t bytes) {
   const size_t n = h->state_fbr ? h->nlp : h->npt;
   QD_HIP(hipMemcpy(p, src, cs * n, hipMemcpyDeviceToHost));
   p += cs * std::max(h->npt, h->nlp);
+  if (ckpt_both(h)) {
+    if (c.state_fbr == 2)
+      QD_HIP(hipMemcpy(p, f64 ? (const void *)h->psi64 : (const void *)h->psi32, cs * h->npt, hipMemcpyDeviceToHost));
+    p += cs * h->npt;
+  }
   if (h->do_crp) {
     const size_t acc = sizeof(double2) * h->nx * h->ne, ring = sizeof(double2) * h->nx * h->tblock;
     QD_HIP(hipMemcpy(p, h->fai, acc, hipMemcpyDeviceToHost));
@@ -2357,6 +2388,8 @@ int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes) {
     return fail(QD_ERR_ARG, "checkpoint does not belong to a handle of this problem and options");
   if (bytes < c.bytes) return fail(QD_ERR_ARG, "checkpoint buffer shorter than its header says");
   if (c.state_fbr && !h->fuse_theta) return fail(QD_ERR_ARG, "checkpoint holds the fused theta state");
+  if (c.state_fbr < 0 || c.state_fbr > 2 || (c.state_fbr == 2 && !ckpt_both(h)))
+    return fail(QD_ERR_ARG, "checkpoint state field out of range");
   QD_HIP(hipSetDevice(h->dev));
   QD_HIP(hipStreamSynchronize(h->stream));
   const char *p = static_cast<const char *>(buf) + sizeof c;
@@ -2365,6 +2398,11 @@ int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes) {
   void *dst = c.state_fbr ? (f64 ? (void *)h->lpsi64 : (void *)h->lpsi32) : (f64 ? (void *)h->psi64 : (void *)h->psi32);
   QD_HIP(hipMemcpy(dst, p, cs * (c.state_fbr ? h->nlp : h->npt), hipMemcpyHostToDevice));
   p += cs * std::max(h->npt, h->nlp);
+  if (ckpt_both(h)) {
+    if (c.state_fbr == 2)
+      QD_HIP(hipMemcpy(f64 ? (void *)h->psi64 : (void *)h->psi32, p, cs * h->npt, hipMemcpyHostToDevice));
+    p += cs * h->npt;
+  }
   if (h->do_crp) {
     const size_t acc = sizeof(double2) * h->nx * h->ne, ring = sizeof(double2) * h->nx * h->tblock;
     QD_HIP(hipMemcpy(h->fai, p, acc, hipMemcpyHostToDevice));
@@ -2379,7 +2417,7 @@ int qd_checkpoint_load(qd_handle *h, const void *buf, int64_t bytes) {
   h->part_valid = false;  // the loaded accumulators' P(E) partials are rebuilt by the next fold
   h->slot = (int)c.slot;
   h->state_fbr = c.state_fbr != 0;
-  h->dvr_valid = !h->state_fbr;
+  h->dvr_valid = c.state_fbr != 1;
   h->pending_prephase = c.pending_prephase != 0;
   return QD_OK;
 }

@@ -1148,7 +1148,8 @@ struct ThetaFusedArgs {
   const long long *dcol_vidx;
   const float *d1f, *d2f;      // dump factors (nullptr: no dump)
   int n1;
-  int r_lo, r_hi;              // flux rows (r_lo > r_hi: none)
+  int r_lo, r_hi;              // psi rows written (r_lo > r_hi: none)
+  int pre_dump;                // 1: the rows as before the dump (CUDA propagator order), 0: after it
   const int4 *items;           // per work group: first block of its group, blocks, tile begin, tile end
 };
 
@@ -1261,7 +1262,8 @@ __global__ __launch_bounds__(NTH) void k_theta_fused(ThetaFusedArgs P) {
           }
       }
     }
-    // DVR domain: dump, flux rows, V phase; then the pieces of the forward B operand
+    // DVR domain: dump, psi rows (before or after the dump), V phase; then the
+    // pieces of the forward B operand
     float dfac = 1.f;
     if (P.d1f && rin) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
     const int rr = row / P.n1;
@@ -1273,7 +1275,9 @@ __global__ __launch_bounds__(NTH) void k_theta_fused(ThetaFusedArgs P) {
       for (int v = 0; v < 4; v++) {
         const int n = nt * 16 + 4 * kg + v;
         const float2 c = make_float2(acc[0][nt][v] * dfac, acc[1][nt][v] * dfac);
-        if (frow) buf_st_f2(c, rP, n < nd ? (unsigned)(row + R * n) * 8u : TM_OOB);
+        if (frow)
+          buf_st_f2(P.pre_dump ? make_float2(acc[0][nt][v], acc[1][nt][v]) : c, rP,
+                    n < nd ? (unsigned)(row + R * n) * 8u : TM_OOB);
         const float2 w = wv[nt][v];
         Xw[rl * XS + n] = make_float2(c.x * w.x - c.y * w.y, c.x * w.y + c.y * w.x);
       }

@@ -9,18 +9,24 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec,graph", [("fp64", 0), ("mixed", 0), ("mixed", 1)])
-def test_checkpoint_resumes_bit_for_bit(qd, prec, graph):
+@pytest.mark.parametrize("prec,graph,variant", [("fp64", 0, "cpu"), ("mixed", 0, "cpu"), ("mixed", 1, "cpu"),
+                                                ("mixed", 0, "cuda"), ("mixed", 1, "cuda")])
+def test_checkpoint_resumes_bit_for_bit(qd, prec, graph, variant):
+    """Mixed precision runs the fused theta step; in the CUDA propagator's
+    order its checkpoint carries psi beside the FBR state, and psi reads back
+    right after a load."""
     p = qd.problem.config(0)
-    print("case: checkpoint", prec, "graph", graph, flush=True)
-    with qd.Propagator(p.copy(), precision=prec, use_graph=graph) as a:
+    print("case: checkpoint", prec, "graph", graph, variant, flush=True)
+    with qd.Propagator(p.copy(), precision=prec, use_graph=graph, variant=variant) as a:
         a.propagate(100)                      # 64-step block closed, 36 slices open
         ck = a.checkpoint()
+        psi_ck = a.psi()
         a.propagate(150)
         psi_a, crp_a, n_a = a.psi(), a.crp(), a.steps()
-    with qd.Propagator(p.copy(), precision=prec, use_graph=graph) as b:
+    with qd.Propagator(p.copy(), precision=prec, use_graph=graph, variant=variant) as b:
         b.restore(ck)
         assert b.steps() == 100
+        assert np.array_equal(b.psi(), psi_ck)
         b.propagate(150)
         psi_b, crp_b, n_b = b.psi(), b.crp(), b.steps()
     assert n_a == n_b == 250

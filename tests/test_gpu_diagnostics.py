@@ -65,6 +65,43 @@ def test_mixed_fused_theta_with_diagnostics(qd, oracle):
     assert np.max(np.abs(crp_b - crp_a)) < 1e-5
 
 
+def test_cuda_order_fused_theta_with_diagnostics(qd):
+    """The fused theta step in the CUDA propagator's order, whose FBR state
+    holds the dumped psi: with diagnostics every 5 steps (each needs psi from
+    the step before it) and a P(E) callback every 7 steps, psi and P(E) stay
+    within the mixed path's rounding of the run without them, and the
+    diagnostics within 1e-5 of the unfused handle's."""
+    p = qd.problem.config(0)
+    print("case: mixed cuda-order diagnostics every 5 steps, 40 steps", flush=True)
+    with qd.Propagator(p.copy(), precision="mixed", variant="cuda") as a:
+        assert a.tuning()["fused_theta"] == 1
+        a.propagate(40)
+        psi_a, crp_a = a.psi(), a.crp()
+    out = {}
+    for fuse in (True, False):
+        with qd.Propagator(p.copy(), precision="mixed", variant="cuda", diagnostics_every=5, crp_every=7,
+                           fuse_theta=fuse) as b:
+            assert b.tuning()["fused_theta"] == (1 if fuse else 0)
+            seen = []
+            b.set_callback(lambda steps, crp: seen.append(steps) or False)
+            b.propagate(23)
+            d_mid = b.diagnostics()
+            b.propagate(17)
+            out[fuse] = (b.diagnostics(), d_mid, b.psi(), b.crp(), seen)
+    d_f, dm_f, psi_f, crp_f, seen_f = out[True]
+    d_u, dm_u, psi_u, crp_u, _ = out[False]
+    assert seen_f == [7, 14, 21, 28, 35]
+    assert d_f["steps"] == 40 and dm_f["steps"] == 20
+    for x, y in ((d_f, d_u), (dm_f, dm_u)):
+        scale = abs(y["e_kin"]) + abs(y["e_rot"]) + abs(y["e_pot"])
+        for n in NAMES:
+            assert abs(x[n] - y[n]) < 1e-5 * (y["module"] if n == "module" else scale), (n, x[n], y[n])
+    assert np.max(np.abs(psi_f - psi_a)) < 1e-5 * np.max(np.abs(psi_a))
+    assert np.max(np.abs(psi_f - psi_u)) < 1e-5 * np.max(np.abs(psi_u))
+    assert np.max(np.abs(crp_f - crp_a)) < 1e-5
+    assert np.max(np.abs(crp_f - crp_u)) < 1e-5
+
+
 def test_total_energy_conserved_without_absorbers(qd, oracle):
     """With the absorbers off the reference's e_tot = e_kin + e_rot + e_pot
     stays constant to the splitting error over the first 50 steps of cfg 0

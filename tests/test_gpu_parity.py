@@ -97,17 +97,18 @@ def test_handle_resume_and_norm(qd, hh2_small):
         assert crp_err(prop.crp(), crp_a) < 1e-12
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec,variant", [("fp64", "cpu"), ("mixed", "cpu"), ("mixed", "cuda")])
 @pytest.mark.parametrize("crp_every", [0, 20])
-def test_graph_replay_equals_eager(qd, hh2_small, prec, crp_every):
+def test_graph_replay_equals_eager(qd, hh2_small, prec, variant, crp_every):
     """Whole flux blocks replayed as one captured graph give the same psi and
     P(E) as the same kernels launched one by one.  Three propagate calls with
     host work between them replay the graph several times; with crp_every = 20
-    some blocks hold a P(E) point and run eagerly, so both kinds interleave."""
-    print("case: graph replay", prec, "crp_every", crp_every, flush=True)
+    some blocks hold a P(E) point and run eagerly, so both kinds interleave.
+    Mixed precision runs the fused theta step, in either propagator order."""
+    print("case: graph replay", prec, variant, "crp_every", crp_every, flush=True)
     out = []
     for graph in (0, 1):
-        with qd.Propagator(hh2_small.copy(), precision=prec, flux_block=16, crp_every=crp_every,
+        with qd.Propagator(hh2_small.copy(), precision=prec, variant=variant, flux_block=16, crp_every=crp_every,
                            use_graph=graph) as prop:
             seen = []
             for n in (40, 33, 55):
@@ -143,27 +144,29 @@ def test_autotuned_handle_matches(qd, oracle, hh2_small, prec):
         assert np.max(np.abs(crp_t - crp_d)) < 1e-5
 
 
+@pytest.mark.parametrize("variant", ["cpu", "cuda"])
 @pytest.mark.parametrize("kind", ["single", "batched"])
-def test_fused_theta_step_matches_unfused(qd, kind):
+def test_fused_theta_step_matches_unfused(qd, kind, variant):
     """The fused theta step (theta-backward, dump, flux rows, V phase and the
     next theta-forward in one kernel; the state between steps is the FBR
     array) against the unfused mixed step, across a resume with a norm read in
-    between: psi (rebuilt from the FBR state) and P(E) agree to the rounding
+    between: psi (rebuilt from the FBR state, or in the CUDA propagator's
+    order written by the step before the read) and P(E) agree to the rounding
     level of the mixed path, and are no further from the FP64 path than the
     unfused ones."""
     pb = qd.problem
-    print("case: fused theta vs unfused,", kind, flush=True)
+    print("case: fused theta vs unfused,", kind, variant, flush=True)
     if kind == "single":
         prob = pb.config(0, steps=60)
-        mk = lambda fuse: qd.Propagator(prob.copy(), precision="mixed", fuse_theta=fuse)
+        mk = lambda fuse: qd.Propagator(prob.copy(), precision="mixed", variant=variant, fuse_theta=fuse)
     else:
         base = pb.config(0, nR=64, nr=32, ntheta=12, steps=60, e0_ev=0.6)
         jk = pb.make_jk_problem(base, pb.H_H2, [0, 3, 7], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
-        mk = lambda fuse: qd.propagator.JKPropagator(jk, precision="mixed", fuse_theta=fuse)
+        mk = lambda fuse: qd.propagator.JKPropagator(jk, precision="mixed", variant=variant, fuse_theta=fuse)
     if kind == "single":
-        mk64 = lambda: qd.Propagator(prob.copy(), precision="fp64")
+        mk64 = lambda: qd.Propagator(prob.copy(), precision="fp64", variant=variant)
     else:
-        mk64 = lambda: qd.propagator.JKPropagator(jk, precision="fp64")
+        mk64 = lambda: qd.propagator.JKPropagator(jk, precision="fp64", variant=variant)
     out = {}
     for fuse in (False, True, "fp64"):
         with (mk64() if fuse == "fp64" else mk(fuse)) as prop:
