@@ -207,7 +207,7 @@ struct qd_handle {
   void *dR = nullptr, *dr = nullptr, *kinR = nullptr, *kinr = nullptr, *twR = nullptr, *twr = nullptr;
   void *Wf = nullptr, *Wb = nullptr;  // packed FP64 [mat][Kpad][Npad] or bf16 [mat][3][Npad][Kpad]
   long long mat_stride = 0;
-  int mat_Kpad = 0, mat_Npad = 0, ncb = 8;
+  int mat_Kpad = 0, mat_Npad = 0, ncb = 8, f64_cb = 8;
   int col_tiles = 1;
   int *bundle_u0 = nullptr;
   RUnit *units = nullptr;
@@ -536,7 +536,11 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   const int rows = (int)((h->n12 + TG_ROWS - 1) / TG_ROWS);
   dim3 grid(rows, h->col_tiles, h->nb);
   if (h->prec == QD_PREC_FP64) {
-#define QD_T64(PR, EP) hipLaunchKernelGGL((k_theta_f64<PR, EP>), grid, dim3(256), 0, h->stream, P)
+#define QD_T64(PR, EP)                                                                       \
+  do {                                                                                       \
+    if (h->f64_cb == 4) hipLaunchKernelGGL((k_theta_f64<PR, EP, 4>), grid, dim3(256), 0, h->stream, P); \
+    else hipLaunchKernelGGL((k_theta_f64<PR, EP, 8>), grid, dim3(256), 0, h->stream, P);    \
+  } while (0)
     if (pro == 0 && epi == 0) QD_T64(0, 0);
     else if (pro == 1 && epi == 0) QD_T64(1, 0);
     else if (pro == 2 && epi == 0) QD_T64(2, 0);
@@ -565,9 +569,9 @@ int launch_theta(qd_handle *h, bool fwd, int pro, int epi) {
   return QD_OK;
 }
 
-// rows = false leaves psi untouched (no flux rows), for the autotuner
-// rows: the flux surface's psi rows; all: every row of psi (the CUDA
-// propagator's order, whose pre-dump psi the FBR state cannot give back)
+// rows: the flux surface's psi rows (false leaves psi untouched, for the
+// autotuner); all: every row of psi (the CUDA propagator's order, whose
+// pre-dump psi the FBR state cannot give back)
 int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   ThetaFusedArgs F;
   F.L = h->lpsi32;
@@ -1253,7 +1257,8 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   }
   // theta matrices: padded shapes shared by all sets
   h->ncb = (h->prec == QD_PREC_MIXED && maxn <= 64) ? 4 : 8;
-  const int colw = h->prec == QD_PREC_FP64 ? TG_COLS : h->ncb * 16;
+  h->f64_cb = maxn <= 64 ? 4 : 8;  // FP64 theta: 64 or 128 columns per work group
+  const int colw = h->prec == QD_PREC_FP64 ? 16 * h->f64_cb : h->ncb * 16;
   h->mat_Npad = (maxn + colw - 1) / colw * colw;
   h->col_tiles = h->mat_Npad / colw;
   const int kc = h->prec == QD_PREC_FP64 ? TG_KC64 : TG_KC16;

@@ -642,12 +642,13 @@ __global__ __launch_bounds__(256) void k_rline(RLineArgs<T> A) {
 // ---------------------------------------------------------------------------
 // Per block: C[2R x N] = A[2R x K] * B[K x N] on psi viewed as a real matrix
 // (W and P are real, LegTransform.F:6-59).  Work group: 64 complex rows
-// (128 real rows) x 128 columns of one block, 4 waves of 32 real rows, K in
-// chunks of 16.  B is packed per matrix as [Kpad][Npad] (row-major).
+// (128 real rows) x 16 CB columns of one block (CB = 8, or 4 when no block
+// has more than 64 columns, so that bases of 41..64 functions waste no half
+// of the products), 4 waves of 32 real rows, K in chunks of 16.  B is packed
+// per matrix as [Kpad][Npad] (row-major, Npad a multiple of 16 CB).
 // Prologue PRO: 1 = multiply A by exp(-i V dt); 2 = by dump * exp(-i V dt).
 // Epilogue EPI: 1 = multiply the result by the dump d1[i] d2[j].
 constexpr int TG_ROWS = 64;    // complex rows per work group
-constexpr int TG_COLS = 128;   // output columns per work group
 constexpr int TG_KC64 = 16;    // K chunk for FP64
 constexpr int TG_LDS64 = 144;  // padded row length (doubles) of the LDS tiles
 
@@ -670,8 +671,10 @@ struct ThetaArgs {
   int tpw;                     // 16-row tiles per work group (matrix-stationary kernel)
 };
 
-template <int PRO, int EPI>
+template <int PRO, int EPI, int CB>
 __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
+  static_assert(CB == 4 || CB == 8, "16 or 8 B rows per thread and chunk");
+  constexpr int COLS = 16 * CB;
   __shared__ __align__(16) double As[TG_KC64][TG_LDS64];
   __shared__ __align__(16) double Bs[TG_KC64][TG_LDS64];
   const BlockDesc bd = P.blocks[blockIdx.z];
@@ -686,20 +689,20 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int row0 = blockIdx.x * TG_ROWS;
-  const int col0 = blockIdx.y * TG_COLS;
+  const int col0 = blockIdx.y * COLS;
   if (col0 >= N) return;
-  d4v acc[2][8];
+  d4v acc[2][CB];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 8; b++) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < CB; b++) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
 
   // The next K chunk is fetched into registers (A with the phase prologue
   // applied) while the current one is on the matrix cores.
   const int cr = t & 63;
   const int r = row0 + cr;
   double2 pa[4];
-  double pb[8];
+  double pb[CB];  // TG_KC64 rows x COLS columns over 256 threads
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int s = 0; s < 4; s++) {
@@ -721,7 +724,7 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
       pa[s] = v;
     }
 #pragma unroll
-    for (int s = 0; s < 8; s++) pb[s] = Bp[(size_t)(k0 + (t >> 7) + 2 * s) * P.Npad + col0 + (t & 127)];
+    for (int s = 0; s < CB; s++) pb[s] = Bp[(size_t)(k0 + t / COLS + (256 / COLS) * s) * P.Npad + col0 + t % COLS];
   };
   // v_mfma_f64_16x16x4f64 returns D[(lane/16) + 4v][lane%16] in register v.
   // Feeding MFMA row i from tile row 4(i%4) + i/4 makes register v of lane l
@@ -733,7 +736,7 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
 #pragma unroll
     for (int s = 0; s < 4; s++) *reinterpret_cast<double2 *>(&As[(t >> 6) + 4 * s][2 * cr]) = pa[s];
 #pragma unroll
-    for (int s = 0; s < 8; s++) Bs[(t >> 7) + 2 * s][t & 127] = pb[s];
+    for (int s = 0; s < CB; s++) Bs[t / COLS + (256 / COLS) * s][t % COLS] = pb[s];
     __syncthreads();
     if (k0 + TG_KC64 < K) fetch(k0 + TG_KC64);
 #pragma unroll
@@ -743,7 +746,7 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
 #pragma unroll
       for (int rb = 0; rb < 2; rb++) a[rb] = As[kr][wave * 32 + rb * 16 + arow];
 #pragma unroll
-      for (int cb = 0; cb < 8; cb++) {
+      for (int cb = 0; cb < CB; cb++) {
         const double b = Bs[kr][cb * 16 + (lane & 15)];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb][cb], 0, 0, 0);
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(256) void k_theta_f64(ThetaArgs P) {
     const int rr = wave * 32 + rb * 16 + 4 * (lane >> 4);
     const int r = row0 + rr / 2;
 #pragma unroll
-    for (int cb = 0; cb < 8; cb++) {
+    for (int cb = 0; cb < CB; cb++) {
       const int col = col0 + cb * 16 + (lane & 15);
       if (col >= N) continue;
       double2 c0 = make_double2(acc[rb][cb][0], acc[rb][cb][1]);
