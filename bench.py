@@ -342,7 +342,7 @@ def main():
                  "mixed precision")
         m = timed_batched(qd, jk, "mixed", K, W, autotune=1)
         prof = timed_batched(qd, jk, "mixed", max(10, K // 10), 2, profile=1, autotune=1)
-        f64 = timed_batched(qd, jk, "fp64", max(10, K // 10), 2)
+        f64 = timed_batched(qd, jk, "fp64", max(10, K // 10), 2, autotune=1)
         plain_ms, plain_per_n, counts = plain_fp64_same_grids(qd, jk, max(10, K // 10), 3)
         step_ms = m["wall_ms_per_step"]
         value = npt / (step_ms * 1e-3)

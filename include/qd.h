@@ -233,7 +233,8 @@ int qd_get_crp_blocks(qd_handle *h, double *crp_out);
  * mixing and kinetic factor; 2: r pass with phases and kinetic factor;
  * 3: theta backward (FBR array -> psi, no dump); 4: forward FFT along R only;
  * 5: forward FFT along r only; 6: inverse FFT along R only; 7: inverse FFT
- * along r only.  The FBR array has n1*n2*(FBR columns) complex values. */
+ * along r only.  The FBR array has n1*n2*(FBR columns) complex values.  With
+ * the split mixing R pass selected, stage 1 uses psi as its scratch array. */
 int qd_debug_stage(qd_handle *h, int32_t stage);
 /* kernel configuration in use: [0] autotuned, [1] theta k-steps of the
  * matrix-stationary kernel (0: tiled kernel), [2] its 16-row tiles per work

@@ -139,3 +139,23 @@ def test_jk_groups_above_32_lines_match_oracle(qd, path):
         prop.propagate(12)
         psi_g = prop.psi().view(np.complex128)
     assert rel(psi_g, psi_o) < (1e-10 if prec == "fp64" else 2e-5)
+
+
+def test_autotuned_batch_keeps_its_wavefunction(qd):
+    """The autotuner times every R-pass candidate on the handle's arrays before
+    the first step; the split pass's trials take a scratch array of their own,
+    so the state the handle starts from is untouched: an autotuned mixed batch
+    (on a grid where the split pass is a candidate) propagates like an
+    untuned one, to the mixed path's rounding."""
+    pb = qd.problem
+    base = pb.config(0, nR=128, nr=16, ntheta=8, steps=30, e0_ev=0.6)
+    jk = pb.make_jk_problem(base, pb.H_H2, [0, 3, 9], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
+    print("case: autotuned batch vs untuned, 128 x 16 grid", flush=True)
+    out = {}
+    for tune in (0, 1):
+        with qd.propagator.JKPropagator(jk, precision="mixed", autotune=tune) as prop:
+            prop.propagate(30)
+            out[tune] = (prop.psi().view(np.complex128), prop.crp_by_J())
+    assert rel(out[1][0], out[0][0]) < 1e-5
+    for J in out[0][1]:
+        assert np.max(np.abs(out[1][1][J] - out[0][1][J])) < 1e-5

@@ -155,6 +155,8 @@ def test_r_pass_mixing_jk(qd, prec, case):
                 prop.debug_select(1, 3)  # three consecutive bundles per turn of a work group
         if case.startswith("split"):
             prop.debug_select(2, 1)
+        elif prop.tuning()["rpass_split_mix"]:
+            prop.debug_select(2, 0)  # FP64 takes the split pass by default where it has no other register-FFT one
         prop.set_fbr(as_f64(y))
         prop.debug_stage(1)
         got = prop.get_fbr().view(np.complex128)
