@@ -205,6 +205,8 @@ struct qd_handle {
   double *d1 = nullptr, *d2 = nullptr;
   float *d1f = nullptr, *d2f = nullptr;
   void *dR = nullptr, *dr = nullptr, *kinR = nullptr, *kinr = nullptr, *twR = nullptr, *twr = nullptr;
+  // shear-form tables of the FP32 register passes (rf_tab<float>); FP64: the ones above
+  void *ltwR = nullptr, *ltwr = nullptr, *lkinR = nullptr, *lkinr = nullptr, *lphR = nullptr, *lphr = nullptr;
   void *Wf = nullptr, *Wb = nullptr;  // packed FP64 [mat][Kpad][Npad] or bf16 [mat][3][Npad][Kpad]
   long long mat_stride = 0;
   int mat_Kpad = 0, mat_Npad = 0, ncb = 8, f64_cb = 8;
@@ -372,6 +374,10 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.tw = (const TB *)h->twR;
   a.kin = (const TB *)h->kinR;
   a.dR = (const TB *)h->dR;
+  constexpr bool f64 = sizeof(T) == 8;  // FP64: the register passes read the plain tables
+  a.ltw = (const typename rf_tab<T>::tw *)(f64 ? h->twR : h->ltwR);
+  a.lkin = (const typename rf_tab<T>::kin *)(f64 ? h->kinR : h->lkinR);
+  a.lph = (const typename rf_tab<T>::ph *)(f64 ? h->dR : h->lphR);
   a.fcol_lam = h->fcol_lam;
   a.bundle_u0 = h->bundle_u0;
   a.units = h->units;
@@ -413,9 +419,9 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
     if (h->rp_mx && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
       RMixArgs m;
       m.data = (float2 *)data;
-      m.tw = (const float4 *)h->twR;
-      m.kin = (const float4 *)h->kinR;
-      m.dR = (const float4 *)h->dR;
+      m.tw = (const float2 *)h->ltwR;
+      m.kin = (const float4 *)h->lkinR;
+      m.dR = (const float4 *)h->lphR;
       m.umat = h->mx_umat;
       m.chunk = h->mx_chunk;
       // the two bundle classes touch disjoint lines
@@ -473,6 +479,10 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.tw = (const TB *)h->twr;
   a.kin = (const TB *)h->kinr;
   a.dr = (const TB *)h->dr;
+  constexpr bool f64 = sizeof(T) == 8;  // FP64: the register pass reads the plain tables
+  a.ltw = (const typename rf_tab<T>::tw *)(f64 ? h->twr : h->ltwr);
+  a.lkin = (const typename rf_tab<T>::kin *)(f64 ? h->kinr : h->lkinr);
+  a.lph = (const typename rf_tab<T>::ph *)(f64 ? h->dr : h->lphr);
   a.fcol_j = h->fcol_j;
   a.n1 = h->n1;
   a.n2 = h->n2;
@@ -950,6 +960,52 @@ int put_table(qd_handle *h, void **dst, const std::vector<double> &v) {
     QD_RC(upload(&d, f.data(), f.size()));
     *dst = d;
   }
+  return QD_OK;
+}
+
+// Shear-form tables of the FP32 register passes along an axis of n points
+// (rf_tab<float> in qd_kernels.hpp, rf_line in qd_rfft.hpp) from the phase
+// angles of the kinetic factors (kin_ang[n]) and of the D/2 rows (ph_ang, rows
+// of n): the twiddles W_n^e for |e| <= n/4; the kinetic factors, with the sign
+// and 1/n in a scale, in the order stage B's thread b reads them (entry
+// b + N2 k1 is frequency b' + N2 k1, b' the centred b); the D/2 rows as the
+// constants (t, s, 2t) of half their angle.  Nothing for a length without a
+// register pass.
+int put_lift_tables(qd_handle *h, int n, const std::vector<double> &kin_ang, const std::vector<double> &ph_ang,
+                    void **tw, void **kin, void **ph) {
+  const int N1 = rf_n1(n);
+  if (N1 == 0) return QD_OK;
+  const int N2 = n / N1;
+  const long double pi = 3.141592653589793238462643383279502884L;
+  std::vector<float> ftw(2 * (size_t)(n / 2 + 1)), fkin(4 * (size_t)n), fph(4 * ph_ang.size());
+  for (int e = -n / 4; e <= n / 4; e++)
+    qdt::lift_consts((double)(-2 * pi * e / n), ftw[2 * (e + n / 4)], ftw[2 * (e + n / 4) + 1]);
+  for (int b = 0; b < N2; b++)
+    for (int k1 = 0; k1 < N1; k1++) {
+      const int bc = b < (N2 + 1) / 2 ? b : b - N2;
+      const int f = ((bc + N2 * k1) % n + n) % n;
+      int sg = 1;
+      const double th = qdt::lift_reduce(kin_ang[f], sg);
+      float *o = &fkin[4 * (size_t)(b + N2 * k1)];
+      qdt::lift_consts(th, o[0], o[1]);
+      const double c = (double)sg / n;
+      o[2] = (float)c;
+      o[3] = (float)(c - (double)o[2]);
+    }
+  const long double twopi = 2 * pi;
+  for (size_t i = 0; i < ph_ang.size(); i++) {
+    float *o = &fph[4 * i];
+    qdt::lift_consts((double)(remainderl((long double)ph_ang[i], twopi) / 2), o[0], o[1]);
+    o[2] = 2 * o[0];
+    o[3] = 0.f;
+  }
+  float *d = nullptr;
+  QD_RC(upload(&d, ftw.data(), ftw.size()));
+  *tw = d;
+  QD_RC(upload(&d, fkin.data(), fkin.size()));
+  *kin = d;
+  QD_RC(upload(&d, fph.data(), fph.size()));
+  *ph = d;
   return QD_OK;
 }
 
@@ -1461,20 +1517,24 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   const int njr = S.jmax + 1;
   std::vector<double> dRt(2 * lams.size() * n1), drt(2 * (size_t)njr * n2), kinR(2 * n1), kinr(2 * n2), twR(2 * n1),
       twr(2 * n2);
+  std::vector<double> dRa(lams.size() * n1), dra((size_t)njr * n2), kRa(n1), kra(n2);  // the phase angles
   for (size_t q = 0; q < lams.size(); q++)
     for (int i = 0; i < n1; i++) {
       const double a = (-dt / 2) * ((double)lams[q] * I1[i]);
+      dRa[q * n1 + i] = a;
       dRt[2 * (q * n1 + i)] = cos(a);
       dRt[2 * (q * n1 + i) + 1] = sin(a);
     }
   for (int j = 0; j < njr; j++)
     for (int r = 0; r < n2; r++) {
       const double a = (-dt / 2) * ((double)j * (j + 1) * I2[r]);
+      dra[(size_t)j * n2 + r] = a;
       drt[2 * ((size_t)j * n2 + r)] = cos(a);
       drt[2 * ((size_t)j * n2 + r) + 1] = sin(a);
     }
   for (int i = 0; i < n1; i++) {
     const double T = p1v[i] * p1v[i] / (2 * p->r1.mass);
+    kRa[i] = -dt * T;
     kinR[2 * i] = cos(-dt * T) / n1;
     kinR[2 * i + 1] = sin(-dt * T) / n1;
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * i / n1;
@@ -1483,6 +1543,7 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   }
   for (int j = 0; j < n2; j++) {
     const double T = p2v[j] * p2v[j] / (2 * p->r2.mass);
+    kra[j] = -dt * T;
     kinr[2 * j] = cos(-dt * T) / n2;
     kinr[2 * j + 1] = sin(-dt * T) / n2;
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * j / n2;
@@ -1495,6 +1556,10 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   QD_RC(put_table(h, &h->kinr, kinr));
   QD_RC(put_table(h, &h->twR, twR));
   QD_RC(put_table(h, &h->twr, twr));
+  if (h->prec == QD_PREC_MIXED) {
+    QD_RC(put_lift_tables(h, n1, kRa, dRa, &h->ltwR, &h->lkinR, &h->lphR));
+    QD_RC(put_lift_tables(h, n2, kra, dra, &h->ltwr, &h->lkinr, &h->lphr));
+  }
   // diagnostics tables: a single J = 0 grid, FBR column c is Legendre degree l = c
   h->diag_ok = S.blocks.size() == 1 && S.blocks[0].J == 0;
   if (h->diag_every > 0 && !h->diag_ok)
@@ -1798,6 +1863,7 @@ void release(qd_handle *h) {
   void *bufs[] = {h->psi64,    h->lpsi64, h->psi32,     h->lpsi32,   h->V,      h->expV32,   h->dcol_vidx,
                   h->dcol_w,   h->fcol_lam, h->fcol_j,  h->d_blocks, h->d_blocks_k, h->d1,   h->d2,       h->d1f,
                   h->d2f,      h->dR,     h->dr,        h->kinR,     h->kinr,   h->twR,      h->twr,
+                  h->ltwR,     h->ltwr,   h->lkinR,     h->lkinr,    h->lphR,   h->lphr,
                   h->Wf,       h->Wb,     h->bundle_u0, h->units,    h->unit_cols, h->umat,  h->bpsi, h->ftau,
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,

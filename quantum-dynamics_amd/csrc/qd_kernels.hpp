@@ -93,6 +93,29 @@ __device__ __forceinline__ float2 tmul(float2 a, float4 w, bool conj_w) {
   r.y = fmaf(a.x, wy, fmaf(a.y, w.x, ty));
   return r;
 }
+// Tables of the register-FFT passes (qd_rfft.hpp).  FP64: the complex values,
+// as above.  FP32: every factor is a rotation applied as three shears
+// (lifting steps, qdt::lift_consts), whose determinant is 1 for any FP32
+// constants, so no table can scale the norm; an entry holds (t, s) =
+// (tan(theta/2), sin theta) with |theta| <= pi/2:
+//   tw   twiddle W_N^e for |e| <= N/4 (centred indices), float2 (t, s)
+//   ph   D/2 phase, any angle theta in (-pi, pi], as two rotations by
+//        theta/2: float4 (t, s, 2t, 0) of theta/2 (the middle shears merge,
+//        five FMAs)
+//   kin  kinetic factor sigma e^{i theta} / n as float4 (t, s, c_hi, c_lo),
+//        sigma = +-1, c = sigma / n, in the order the threads read it
+template <typename T> struct rf_tab;
+template <> struct rf_tab<double> {
+  typedef double2 tw;
+  typedef double2 kin;
+  typedef double2 ph;
+};
+template <> struct rf_tab<float> {
+  typedef float2 tw;
+  typedef float4 kin;
+  typedef float4 ph;
+};
+
 // x * s for a real constant given as hi + lo (FP32) or a double
 __device__ __forceinline__ float csmul(float x, float hi, float lo) { return fmaf(x, hi, x * lo); }
 __device__ __forceinline__ double csmul(double x, double hi, double) { return x * hi; }
@@ -398,6 +421,10 @@ struct RPassArgs {
   const long long *line_off;           // element offset of each line
   const int *line_lam;                 // D_R row of each line
   int nbundle;
+  // the register-FFT kernels' tables (rf_tab; the FP64 ones are tw, kin, dR)
+  const typename rf_tab<T>::tw *ltw;
+  const typename rf_tab<T>::kin *lkin;
+  const typename rf_tab<T>::ph *lph;
 };
 
 template <typename T>
@@ -559,6 +586,10 @@ struct RLineArgs {
   int nlines;  // n1 * FBR columns
   int flags;
   FftPlanArg plan;
+  // the register-FFT kernel's tables (rf_tab; the FP64 ones are tw, kin, dr)
+  const typename rf_tab<T>::tw *ltw;
+  const typename rf_tab<T>::kin *lkin;
+  const typename rf_tab<T>::ph *lph;
 };
 
 template <typename T>

@@ -11,10 +11,12 @@
 // so the forward transform, the kinetic phase and the inverse transform of a
 // line cost two LDS exchanges, and every thread ends with the elements it
 // loaded (the same global addresses, which keeps the store coalesced).
-// The small DFTs are unrolled at compile time with exact constants; in FP32
-// every non-trivial constant and table twiddle is applied as hi + lo (the
-// rounding of a single-float twiddle biases |W| and, applied coherently every
-// step, the norm).
+// The small DFTs are unrolled at compile time with exact constants.  In FP32
+// every twiddle and phase factor is a rotation applied as three shears, whose
+// determinant is 1 whatever the rounding of its constants: a single-float
+// complex factor would bias |W| and, applied coherently every step, the norm,
+// and a hi + lo pair avoids that at 8 instead of 3 FMAs.  The radix-3 and -5
+// constants, which are not rotations, stay hi + lo.
 #pragma once
 
 #include <type_traits>
@@ -83,7 +85,93 @@ constexpr double rf_trig(int e, int S, bool want_sin) {
   return want_sin ? ss : cc;
 }
 
-// x * W_S^E with W = exp(-2 pi i / S) (FWD) or its conjugate.
+// ---- rotations as three shears (FP32) ---------------------------------------
+// x1 = x - t y, y' = y + s x1, x' = x1 - t y' rotates x + iy by theta for
+// t = tan(theta/2), s = sin theta.  Its matrix has determinant 1 for any FP32
+// t and s, so the rounding of the constants moves the angle but never scales
+// the norm, and (-t, -s) undoes (t, s) exactly: three FMAs against the eight
+// of a hi + lo complex product.  The constants are picked like
+// qdt::lift_consts on the host: among the floats within 8 ulp of tan(theta/2),
+// each with the three floats nearest (1 - cos theta) / t, the pair whose
+// product is nearest 1 - cos theta (the composite's trace fixes its angle).
+struct rf_lift_c {
+  float t, s;
+};
+constexpr float rf_fstep(float f, int k) {  // f > 0 moved by k float spacings
+  return __builtin_bit_cast(float, (unsigned)((int)__builtin_bit_cast(unsigned, f) + k));
+}
+constexpr double rf_fabs(double x) { return x < 0 ? -x : x; }
+constexpr rf_lift_c rf_lift(double theta) {  // |theta| <= pi/2
+  if (theta == 0) return {0.f, 0.f};
+  const double th = theta < 0 ? -theta : theta;
+  const double sh = rf_sin_small(th / 2), ch = rf_cos_small(th / 2);
+  const double a = 2 * sh * sh;
+  const float t0 = (float)(sh / ch);
+  float tb = t0, sb = (float)(2 * sh * ch);
+  double best = 1e300;
+  for (int k = -8; k <= 8; k++) {
+    const float t = rf_fstep(t0, k);
+    if (!(t > 0)) continue;
+    const float s0 = (float)(a / t);
+    for (int q = -1; q <= 1; q++) {
+      const float s = rf_fstep(s0, q);
+      const double e = rf_fabs((double)t * (double)s - a);
+      if (e < best) {
+        best = e;
+        tb = t;
+        sb = s;
+      }
+    }
+  }
+  return theta < 0 ? rf_lift_c{-tb, -sb} : rf_lift_c{tb, sb};
+}
+
+__device__ __forceinline__ float2 lrot(float2 v, float t, float s) {
+  const float x1 = fmaf(-t, v.y, v.x);
+  const float y1 = fmaf(s, x1, v.y);
+  float2 r;
+  r.x = fmaf(-t, y1, x1);
+  r.y = y1;
+  return r;
+}
+
+// Table factors of the register passes (rf_tab in qd_kernels.hpp): twiddle
+// (inv: its conjugate), D/2 phase, kinetic factor of a length-N line.
+__device__ __forceinline__ double2 rf_tw(double2 v, double2 w, bool inv) { return tmul(v, w, inv); }
+__device__ __forceinline__ float2 rf_tw(float2 v, float2 w, bool inv) {
+  return inv ? lrot(v, -w.x, -w.y) : lrot(v, w.x, w.y);
+}
+__device__ __forceinline__ double2 rf_ph(double2 v, double2 w) { return tmul(v, w, false); }
+// two rotations by theta/2 (t, s, 2t): |theta| up to pi with |t| <= 1
+__device__ __forceinline__ float2 rf_ph(float2 v, float4 w) {
+  const float x1 = fmaf(-w.x, v.y, v.x);
+  const float y1 = fmaf(w.y, x1, v.y);
+  const float x2 = fmaf(-w.z, y1, x1);
+  const float y2 = fmaf(w.y, x2, y1);
+  float2 r;
+  r.x = fmaf(-w.x, y2, x2);
+  r.y = y2;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ double2 rf_kin(double2 v, double2 w) {
+  return tmul(v, w, false);
+}
+template <int N>
+__device__ __forceinline__ float2 rf_kin(float2 v, float4 w) {
+  float2 u;
+  if constexpr ((N & (N - 1)) == 0) {  // c = +-1/N, exact
+    u.x = v.x * w.z;
+    u.y = v.y * w.z;
+  } else {                             // c as hi + lo
+    u.x = fmaf(v.x, w.z, v.x * w.w);
+    u.y = fmaf(v.y, w.z, v.y * w.w);
+  }
+  return lrot(u, w.x, w.y);
+}
+
+// x * W_S^E with W = exp(-2 pi i / S) (FWD) or its conjugate.  FP32: the
+// nearest quarter turn exactly, the rest (|angle| <= pi/4) as three shears.
 template <typename T, int S, int E, bool FWD>
 __device__ __forceinline__ typename cx_of<T>::type rf_twc(typename cx_of<T>::type x) {
   typedef typename cx_of<T>::type C;
@@ -115,12 +203,23 @@ __device__ __forceinline__ typename cx_of<T>::type rf_twc(typename cx_of<T>::typ
       r.x = x.x * wr - x.y * wi;
       r.y = x.x * wi + x.y * wr;
     } else {
-      constexpr float hr = (float)wr, hi = (float)wi;
-      constexpr float lr = (float)(wr - (double)hr), li = (float)(wi - (double)hi);
-      const float tx = x.x * lr - x.y * li;
-      const float ty = x.x * li + x.y * lr;
-      r.x = fmaf(x.x, hr, fmaf(-x.y, hi, tx));
-      r.y = fmaf(x.x, hi, fmaf(x.y, hr, ty));
+      constexpr int q = (8 * e + S) / (2 * S);  // nearest quarter turn
+      constexpr int rn = 4 * e - q * S;         // the rest, in units of 2 pi / (4 S)
+      constexpr rf_lift_c L = rf_lift((FWD ? -2.0 : 2.0) * rf_pi * rn / (4.0 * S));
+      const C y = rn == 0 ? x : lrot(x, L.t, L.s);
+      constexpr int qi = FWD ? (4 - q % 4) % 4 : q % 4;  // (-i)^q forward, i^q inverse, as a power of i
+      if constexpr (qi == 0) {
+        r = y;
+      } else if constexpr (qi == 1) {
+        r.x = -y.y;
+        r.y = y.x;
+      } else if constexpr (qi == 2) {
+        r.x = -y.x;
+        r.y = -y.y;
+      } else {
+        r.x = y.y;
+        r.y = -y.x;
+      }
     }
     return r;
   }
@@ -172,17 +271,26 @@ constexpr bool rf_length(int n) { return rf_n1(n) != 0; }
 
 // Stages A .. B' of one line (see the file comment).  v: the thread's N2
 // points on entry and exit; xidx(n, k): LDS slot of exchange element (n, k)
-// of this line; twa[a * (N2 + 1) + k] = W_N^(a k); kin[k]: kinetic factor.
+// of this line; twa[a * (N2 + 1) + k] = W_N^(a' k'); kin[b + N2 k]: kinetic
+// factor of the frequency stage B's thread b holds in u[k].
+//
+// FP32 runs the decomposition on centred indices, a' = a - N1 [a >= N1/2],
+// k' = k - N2 [k >= (N2 + 1)/2]: with n = a' + N1 m and k = k' + N2 k1 every
+// point and frequency is still met once (mod N), the length-N1 and length-N2
+// DFTs are unchanged (their exponents only matter mod N1, N2), and the table
+// twiddles W_N^(a' k') stay within a quarter turn, which the three-shear
+// rotation needs.  So thread a holds point a' + N1 m in v[m] (rf_pos) and the
+// kinetic table is stored in the threads' order.  FP64 keeps a' = a, k' = k.
 template <typename T, int N1, int N2, typename XI>
 __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool act, typename cx_of<T>::type *xb,
-                                        XI xidx, const typename tab_of<T>::type *twa,
-                                        const typename tab_of<T>::type *kin) {
+                                        XI xidx, const typename rf_tab<T>::tw *twa,
+                                        const typename rf_tab<T>::kin *kin) {
   typedef typename cx_of<T>::type C;
   constexpr int XS = N2 + 1;
   if (act) {
     rdft<T, N2, true>(v);
 #pragma unroll
-    for (int k = 1; k < N2; k++) v[k] = tmul(v[k], twa[a * XS + k], false);
+    for (int k = 1; k < N2; k++) v[k] = rf_tw(v[k], twa[a * XS + k], false);
 #pragma unroll
     for (int k = 0; k < N2; k++) xb[xidx(a, k)] = v[k];
   }
@@ -193,10 +301,10 @@ __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool 
     for (int n = 0; n < N1; n++) u[n] = xb[xidx(n, a)];
     rdft<T, N1, true>(u);
 #pragma unroll
-    for (int k = 0; k < N1; k++) u[k] = tmul(u[k], kin[a + N2 * k], false);
+    for (int k = 0; k < N1; k++) u[k] = rf_kin<N1 * N2>(u[k], kin[a + N2 * k]);
     rdft<T, N1, false>(u);
 #pragma unroll
-    for (int c = 1; c < N1; c++) u[c] = tmul(u[c], twa[c * XS + a], true);
+    for (int c = 1; c < N1; c++) u[c] = rf_tw(u[c], twa[c * XS + a], true);
 #pragma unroll
     for (int c = 0; c < N1; c++) xb[xidx(c, a)] = u[c];
   }
@@ -208,19 +316,37 @@ __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool 
   }
 }
 
-template <int N1, int N2>
-constexpr size_t rf_table_bytes(size_t tabsz) {
-  return ((size_t)N1 * (N2 + 1) + (size_t)N1 * N2) * tabsz;
+// Point of the line held in element m of thread a (see rf_line).
+template <int N1, int N2, bool CEN>
+struct rf_pos {
+  int pb, p0;
+  __device__ __forceinline__ explicit rf_pos(int a) {
+    const bool up = CEN && a >= N1 / 2;
+    pb = up ? a - N1 : a;
+    p0 = up ? a - N1 + N1 * N2 : a;
+  }
+  __device__ __forceinline__ int operator()(int m) const { return m == 0 ? p0 : pb + N1 * m; }
+};
+
+template <typename T, int N1, int N2>
+constexpr size_t rf_table_bytes() {
+  return (size_t)N1 * (N2 + 1) * sizeof(typename rf_tab<T>::tw) + (size_t)N1 * N2 * sizeof(typename rf_tab<T>::kin);
 }
 
-// Twiddle and kinetic tables into LDS.
+// Twiddle and kinetic tables into LDS.  tw: FP64 W_N^e for e < N; FP32 the
+// shear constants of W_N^e for -N/4 <= e <= N/4 at tw[e + N/4].
 template <typename T, int N1, int N2>
-__device__ __forceinline__ void rf_tables(typename tab_of<T>::type *twa, typename tab_of<T>::type *kin,
-                                          const typename tab_of<T>::type *tw, const typename tab_of<T>::type *kg) {
+__device__ __forceinline__ void rf_tables(typename rf_tab<T>::tw *twa, typename rf_tab<T>::kin *kin,
+                                          const typename rf_tab<T>::tw *tw, const typename rf_tab<T>::kin *kg) {
   constexpr int N = N1 * N2, XS = N2 + 1;
   for (int e = threadIdx.x; e < N; e += blockDim.x) {
     const int aa = e / N2, kk = e % N2;
-    twa[aa * XS + kk] = tw[aa * kk];
+    if constexpr (sizeof(T) == 4) {
+      const int ac = aa < N1 / 2 ? aa : aa - N1, kc = kk < (N2 + 1) / 2 ? kk : kk - N2;
+      twa[aa * XS + kk] = tw[ac * kc + N / 4];
+    } else {
+      twa[aa * XS + kk] = tw[aa * kk];
+    }
     kin[e] = kg[e];
   }
 }
@@ -291,17 +417,20 @@ __device__ __forceinline__ void rf_mix_bundle(typename cx_of<T>::type *stg, int 
 template <typename T, int N1, int N2, bool MIXT, int LW>
 __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
   typedef typename cx_of<T>::type C;
-  typedef typename tab_of<T>::type TB;
+  typedef typename rf_tab<T>::tw TW;
+  typedef typename rf_tab<T>::kin TK;
+  typedef typename rf_tab<T>::ph TB;
   constexpr int N = N1 * N2, NTH = LW * N1, XS = N2 + 1, LX = N1 * XS;
   constexpr int SP = N + N / 32 + 1;  // staged line stride
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
-  TB *s_kin = s_twa + N1 * XS;
+  TW *s_twa = reinterpret_cast<TW *>(smem_raw);
+  TK *s_kin = reinterpret_cast<TK *>(s_twa + N1 * XS);
   C *xch = reinterpret_cast<C *>(s_kin + N);  // [LW][LX]
   C *stg = xch + LW * LX;                       // [LW][SP] (mixing) or the D_R rows
   const int t = threadIdx.x;
   const int a = t % N1, L = t / N1;
-  rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
+  const rf_pos<N1, N2, sizeof(T) == 4> P(a);
+  rf_tables<T, N1, N2>(s_twa, s_kin, A.ltw, A.lkin);
   C v[N2];
   C *xb = xch + L * LX;
   auto xidx = [](int n, int k) { return n * XS + k; };
@@ -323,9 +452,9 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
     // memory (the loop body stays branch-free).
     auto fetch = [&](long long b, int lam, bool has) {
       const __amdgpu_buffer_rsrc_t rd = buf_rsrc(A.data + b * LW * N, has ? (unsigned)(LW * N * sizeof(C)) : 0u);
-      const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.dR + (size_t)lam * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
+      const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.lph + (size_t)lam * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
 #pragma unroll
-      for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)((L * N + a + N1 * m) * sizeof(C)));
+      for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)((L * N + P(m)) * sizeof(C)));
 #pragma unroll
       for (int q = 0; q < PHE; q++)
         if (N % NTH == 0 || q * NTH + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * NTH + t) * sizeof(TB)));
@@ -346,14 +475,14 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
         if (N % NTH == 0 || q * NTH + t < N) ph[q * NTH + t] = phr[q];
       __syncthreads();  // row (and, first time round, the tables)
 #pragma unroll
-      for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
+      for (int m = 0; m < N2; m++) v[m] = rf_ph(raw[m], ph[P(m)]);
       // unconditional (empty descriptors past the last batch) keeps the loop
       // body straight-line, so the waits stay exact across iterations
       fetch(bn, lam_n, has_n);
       rf_line<T, N1, N2>(v, a, true, xb, xidx, s_twa, s_kin);
       C *dst = A.data + (b * LW + L) * N;
 #pragma unroll
-      for (int m = 0; m < N2; m++) dst[a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+      for (int m = 0; m < N2; m++) dst[P(m)] = rf_ph(v[m], ph[P(m)]);
       buf ^= 1;
     }
   } else {
@@ -428,15 +557,15 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
         __syncthreads();
       }
       const bool act = L < nl;
-      const TB *ph = A.dR + (size_t)(act ? s_lam[L] : 0) * N;
+      const TB *ph = A.lph + (size_t)(act ? s_lam[L] : 0) * N;
       if (act) {
 #pragma unroll
-        for (int m = 0; m < N2; m++) v[m] = tmul(stg[L * SP + a + N1 * m], ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) v[m] = rf_ph(stg[L * SP + P(m)], ph[P(m)]);
       }
       rf_line<T, N1, N2>(v, a, act, xb, xidx, s_twa, s_kin);
       if (act) {
 #pragma unroll
-        for (int m = 0; m < N2; m++) stg[L * SP + a + N1 * m] = tmul(v[m], ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) stg[L * SP + P(m)] = rf_ph(v[m], ph[P(m)]);
       }
       __syncthreads();
       // l -> K: c_K = sum_m U[K, m] c_m
@@ -456,10 +585,9 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
 template <typename T, int N1, int N2, bool MIXT, int LW>
 constexpr size_t rpass_rf_lds() {
   constexpr int N = N1 * N2;
-  return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
-         (size_t)LW * N1 * (N2 + 1) * sizeof(typename cx_of<T>::type) +
+  return rf_table_bytes<T, N1, N2>() + (size_t)LW * N1 * (N2 + 1) * sizeof(typename cx_of<T>::type) +
          (MIXT ? (size_t)LW * (N + N / 32 + 1) * sizeof(typename cx_of<T>::type)
-               : (size_t)2 * N * sizeof(typename tab_of<T>::type));
+               : (size_t)2 * N * sizeof(typename rf_tab<T>::ph));
 }
 
 // ---- mixing R pass with prefetch (FP32) ------------------------------------
@@ -488,9 +616,9 @@ constexpr int MX_CHUNK = 8;  // default (on the J = 0..20 batch: 1, 2, 4, 8, 16 
 
 struct RMixArgs {
   float2 *data;
-  const float4 *tw;          // exp(-2 pi i e / n1), hi + lo
-  const float4 *kin;         // exp(-i T1 dt) / n1
-  const float4 *dR;          // D_R/2 rows [lam][n1]
+  const float2 *tw;          // W_n1^e, |e| <= n1/4, as shear constants (rf_tab<float>)
+  const float4 *kin;         // exp(-i T1 dt) / n1, in the threads' order (rf_tab<float>)
+  const float4 *dR;          // D_R/2 rows [lam][n1] (rf_tab<float>)
   const MxHead *head;        // [nbundle]
   const long long *line_off;  // [nbundle][NL] element offset of each line
   const int *line_lam;       // [nbundle][NL] D_R row of each line
@@ -507,7 +635,7 @@ constexpr int mx_row_stride() {
 }
 template <int N1, int N2, int NL>
 constexpr size_t rpass_mx_lds() {
-  return rf_table_bytes<N1, N2>(sizeof(float4)) + (size_t)NL * mx_row_stride<N1, N2, NL>() * sizeof(float2);
+  return rf_table_bytes<float, N1, N2>() + (size_t)NL * mx_row_stride<N1, N2, NL>() * sizeof(float2);
 }
 
 // Unit matrices are read through the constant address space: their addresses
@@ -614,12 +742,13 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
   static_assert(N % 64 == 0 && NT % N == 0, "the points of a unit must fill whole waves");
   static_assert(GB % 4 == 0 && GB <= NL && NL <= 64, "unit size bucket");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
-  TB *s_kin = s_twa + N1 * XS;
+  float2 *s_twa = reinterpret_cast<float2 *>(smem_raw);
+  float4 *s_kin = reinterpret_cast<float4 *>(s_twa + N1 * XS);
   C *stg = reinterpret_cast<C *>(s_kin + N);          // [NL][SP]
   __shared__ long long s_off[NL];
   __shared__ int s_unit[NL];
   const int t = threadIdx.x, a = t % N1, L = t / N1;
+  const rf_pos<N1, N2, true> P(a);
   rf_tables<float, N1, N2>(s_twa, s_kin, A.tw, A.kin);
 
   // descriptors of a bundle (registers): header, this thread's line, unit t
@@ -729,16 +858,16 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
         const TB *ph = A.dR + (size_t)lam * N;
         if constexpr (PREF) {
 #pragma unroll
-          for (int m = 0; m < N2; m++) phv[m] = ph[a + N1 * m];
+          for (int m = 0; m < N2; m++) phv[m] = ph[P(m)];
 #pragma unroll
-          for (int m = 0; m < N2; m++) v[m] = tmul(row[a + N1 * m], phv[m], false);
+          for (int m = 0; m < N2; m++) v[m] = rf_ph(row[P(m)], phv[m]);
         } else {
           // four rows at a time: the loads are not all hoisted ahead, which
           // keeps the pass within its register budget without spilling
 #pragma unroll
           for (int m0 = 0; m0 < N2; m0 += 4) {
 #pragma unroll
-            for (int m = m0; m < m0 + 4; m++) v[m] = tmul(row[a + N1 * m], ph[a + N1 * m], false);
+            for (int m = m0; m < m0 + 4; m++) v[m] = rf_ph(row[P(m)], ph[P(m)]);
             __builtin_amdgcn_sched_barrier(0);
           }
         }
@@ -747,7 +876,7 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
       if (act) {
         const TB *ph = A.dR + (size_t)lam * N;
 #pragma unroll
-        for (int m = 0; m < N2; m++) row[a + N1 * m] = tmul(v[m], PREF ? phv[PREF ? m : 0] : ph[a + N1 * m], false);
+        for (int m = 0; m < N2; m++) row[P(m)] = rf_ph(v[m], PREF ? phv[PREF ? m : 0] : ph[P(m)]);
       }
     }
     __syncthreads();
@@ -798,15 +927,18 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   typedef typename cx_of<T>::type C;
-  typedef typename tab_of<T>::type TB;
+  typedef typename rf_tab<T>::tw TW;
+  typedef typename rf_tab<T>::kin TK;
+  typedef typename rf_tab<T>::ph TB;
   constexpr int N = N1 * N2, LPB = 256 / N1, LS = LPB + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TB *s_twa = reinterpret_cast<TB *>(smem_raw);
-  TB *s_kin = s_twa + N1 * (N2 + 1);
+  TW *s_twa = reinterpret_cast<TW *>(smem_raw);
+  TK *s_kin = reinterpret_cast<TK *>(s_twa + N1 * (N2 + 1));
   C *xch = reinterpret_cast<C *>(s_kin + N);  // [N1][N2][LS]
   const int t = threadIdx.x;
   const int L = t % LPB, a = t / LPB;
-  rf_tables<T, N1, N2>(s_twa, s_kin, A.tw, A.kin);
+  const rf_pos<N1, N2, sizeof(T) == 4> P(a);
+  rf_tables<T, N1, N2>(s_twa, s_kin, A.ltw, A.lkin);
   const size_t st = A.n1;
   // persistent: batch b = LPB consecutive i of one column (n1 % LPB == 0);
   // the next batch's points and D_r row are loaded while this one is
@@ -823,9 +955,9 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
     const C *base = A.data + (size_t)(l0 / A.n1) * A.n1 * N + l0 % A.n1;
     const __amdgpu_buffer_rsrc_t rd =
         buf_rsrc(base, has ? (unsigned)(((size_t)(N - 1) * A.n1 + LPB) * sizeof(C)) : 0u);
-    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.dr + (size_t)jj * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(A.lph + (size_t)jj * N, has ? (unsigned)(N * sizeof(TB)) : 0u);
 #pragma unroll
-    for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)(((size_t)(a + N1 * m) * st + L) * sizeof(C)));
+    for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)(((size_t)P(m) * st + L) * sizeof(C)));
 #pragma unroll
     for (int q = 0; q < PHE; q++)
       if (N % 256 == 0 || q * 256 + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * 256 + t) * sizeof(TB)));
@@ -846,21 +978,20 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
     __syncthreads();  // row (and, first time round, the tables)
     const int line = b * LPB + L;
 #pragma unroll
-    for (int m = 0; m < N2; m++) v[m] = tmul(raw[m], ph[a + N1 * m], false);
+    for (int m = 0; m < N2; m++) v[m] = rf_ph(raw[m], ph[P(m)]);
     fetch(bn, j_n, has_n);
     rf_line<T, N1, N2>(v, a, true, xch, xidx, s_twa, s_kin);
     C *dst = A.data + (size_t)(line / A.n1) * A.n1 * N + line % A.n1;
 #pragma unroll
-    for (int m = 0; m < N2; m++) dst[(size_t)(a + N1 * m) * st] = tmul(v[m], ph[a + N1 * m], false);
+    for (int m = 0; m < N2; m++) dst[(size_t)P(m) * st] = rf_ph(v[m], ph[P(m)]);
     buf ^= 1;
   }
 }
 
 template <typename T, int N1, int N2>
 constexpr size_t rline_rf_lds() {
-  return rf_table_bytes<N1, N2>(sizeof(typename tab_of<T>::type)) +
-         (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type) +
-         (size_t)2 * N1 * N2 * sizeof(typename tab_of<T>::type);
+  return rf_table_bytes<T, N1, N2>() + (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type) +
+         (size_t)2 * N1 * N2 * sizeof(typename rf_tab<T>::ph);
 }
 
 }  // namespace qd

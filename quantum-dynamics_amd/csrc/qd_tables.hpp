@@ -227,6 +227,57 @@ inline void round_unit_complex(double c, double s, float &fc, float &fs) {
     }
 }
 
+// Rotation by theta as three shears (lifting steps),
+//   x1 = x - t y,   y' = y + s x1,   x' = x1 - t y',   t = tan(theta/2), s = sin theta,
+// whose matrix [[1 - ts, -t(2 - st)], [s, 1 - st]] has determinant 1 whatever
+// t and s are: FP32 constants change the angle slightly but never scale the
+// norm, and the rotation by -theta with (-t, -s) is its exact inverse.  The
+// composite's angle follows from its trace, cos = 1 - t s, so (t, s) is chosen
+// among the floats within 8 ulp of tan(theta/2) (each with the three floats
+// nearest (1 - cos theta) / t) to make t s closest to 2 sin^2(theta/2).
+// |theta| <= pi/2 keeps |t| <= 1 and the intermediate values bounded.
+inline void lift_consts(double theta, float &t, float &s) {
+  t = s = 0.f;
+  if (theta == 0.0) return;
+  const bool neg = theta < 0;
+  const ld th = neg ? -(ld)theta : (ld)theta;
+  const ld sh = sinl(th / 2), ch = cosl(th / 2);
+  const ld a = 2 * sh * sh, tex = sh / ch;
+  float t0 = (float)tex, tb = t0, sb = (float)(2 * sh * ch);
+  ld best = INFINITY;
+  for (int k = 0; k < 8; k++) t0 = std::nextafter(t0, -INFINITY);
+  for (int k = 0; k <= 16; k++, t0 = std::nextafter(t0, INFINITY)) {
+    if (!(t0 > 0)) continue;
+    const float s0 = (float)(a / t0);
+    const float cs[3] = {std::nextafter(s0, -INFINITY), s0, std::nextafter(s0, INFINITY)};
+    for (float sc : cs) {
+      const ld e = fabsl((ld)t0 * sc - a);
+      if (e < best) {
+        best = e;
+        tb = t0;
+        sb = sc;
+      }
+    }
+  }
+  t = neg ? -tb : tb;
+  s = neg ? -sb : sb;
+}
+
+// theta reduced to (-pi/2, pi/2]: exp(i theta) = sigma exp(i theta_r), sigma = +-1.
+inline double lift_reduce(double theta, int &sigma) {
+  const ld pi = 3.141592653589793238462643383279502884L;
+  ld r = remainderl((ld)theta, 2 * pi);  // [-pi, pi]
+  sigma = 1;
+  if (r > pi / 2) {
+    r -= pi;
+    sigma = -1;
+  } else if (r <= -pi / 2) {
+    r += pi;
+    sigma = -1;
+  }
+  return (double)r;
+}
+
 // FP32 copy of an orthogonal g x g matrix U (row K, column m), rounded so that
 // every column keeps unit length to second order.  The mixing applies U^T and
 // then U to the same point every step; with plain rounding, U_f = U + E, the
