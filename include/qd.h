@@ -241,7 +241,8 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * group, [3] register-FFT R pass, [4] register-FFT r pass, [5] R-pass lines
  * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
- * [9] its unit-size bucket, [10] fused theta step in use, [11] its 16-row
+ * [9] its unit-size bucket, [10] fused theta step in use (1: the dense
+ * transforms, 2: their parity halves), [11] its 16-row
  * tiles per work group, [12] that mixing R pass runs two work groups per CU,
  * [13] threads per work group of the fused theta step, [14] the split mixing
  * R pass (K -> l mixing, the register FFT pass without mixing, l -> K mixing,
@@ -252,7 +253,10 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * groups per CU; QD_ERR_ARG when the handle does not have that kernel.
  * what 1: its consecutive bundles per turn of a work group (1..64, default 8).
  * what 2: the split mixing R pass on (value 1) or off (value 0); QD_ERR_ARG
- * when the handle does not have it, or has no other R pass for its groups. */
+ * when the handle does not have it, or has no other R pass for its groups.
+ * what 3: the fused theta step on the dense transforms (value 0) or on their
+ * parity halves with 512 (1) or 768 (2) threads per work group; QD_ERR_ARG
+ * when the handle has no fused step or no parity halves. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

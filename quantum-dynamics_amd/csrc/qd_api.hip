@@ -243,6 +243,10 @@ struct qd_handle {
   bool fuse_theta = false, state_fbr = false, dvr_valid = true;
   int fuse_opt = 0, tfu_tpw = 1, tfu_nth = TMX_THREADS;  // tile-blocks per item, threads per work group
   size_t tfu_lds = 0;
+  // the fused step on the parity halves of the theta transforms (k_theta_fused_par)
+  bool tfu_par = false, par_ok = false;
+  void *Wpar = nullptr;
+  long long par_stride = 0;
   // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
   // matrices and the V columns, and the work items built from them
   std::vector<std::pair<int, int>> tfu_groups;  // first block, blocks
@@ -586,9 +590,9 @@ int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   ThetaFusedArgs F;
   F.L = h->lpsi32;
   F.psi = h->psi32;
-  F.Bb = h->Wb;
+  F.Bb = h->tfu_par ? h->Wpar : h->Wb;
   F.Bf = h->Wf;
-  F.mat_stride = h->mat_stride;
+  F.mat_stride = h->tfu_par ? h->par_stride : h->mat_stride;
   F.blocks = h->d_blocks_k;
   F.R = (int)h->n12;
   F.expV = h->expV32;
@@ -601,7 +605,8 @@ int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   F.pre_dump = h->variant == QD_VARIANT_CUDA ? 1 : 0;
   F.items = h->d_tfu_items;
   size_t lds = 0;
-  const void *fn = theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
+  const void *fn =
+      h->tfu_par ? theta_fused_par_kernel(h->tfu_nth, &lds) : theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
   if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
   if (h->tfu_nitems < 1) return fail(QD_ERR_ARG, "fused theta work items not set up");
   dim3 grid((unsigned)h->tfu_nitems, 1, 1);
@@ -1007,6 +1012,46 @@ int put_lift_tables(qd_handle *h, int n, const std::vector<double> &kin_ang, con
   QD_RC(upload(&d, fph.data(), fph.size()));
   *ph = d;
   return QD_OK;
+}
+
+// Parity halves of the theta matrices for k_theta_fused_par: per set
+// [4][3][32][32] BF16 pieces (backward even, backward odd, forward even,
+// forward odd; k contiguous).  False when a half has more than 32 functions or
+// points, or a set is not symmetric under x -> -x with FBR parity (-1)^c to
+// 1e-10 of its largest entry (the kernel relies on it).
+bool pack_theta_parity(const std::vector<MatSet> &mats, std::vector<uint16_t> &out, long long &stride) {
+  constexpr int NP = 32, KP = 32;
+  stride = 4LL * 3 * NP * KP;
+  out.assign(mats.size() * (size_t)stride, 0);
+  for (size_t s = 0; s < mats.size(); s++) {
+    const MatSet &M = mats[s];
+    const int nd = M.nd, nf = M.nf, NH = (nd + 1) / 2, CE = (nf + 1) / 2, CO = nf / 2;
+    if (NH > NP || CE > KP) return false;
+    double mf = 0, mb = 0;
+    for (double v : M.Wf) mf = std::max(mf, std::fabs(v));
+    for (double v : M.Pb) mb = std::max(mb, std::fabs(v));
+    for (int c = 0; c < nf; c++)
+      for (int i = 0; i < nd; i++) {
+        const double sg = (c & 1) ? -1.0 : 1.0;
+        if (std::fabs(M.Pb[(size_t)c * nd + nd - 1 - i] - sg * M.Pb[(size_t)c * nd + i]) > 1e-10 * mb) return false;
+        if (std::fabs(M.Wf[(size_t)(nd - 1 - i) * nf + c] - sg * M.Wf[(size_t)i * nf + c]) > 1e-10 * mf) return false;
+      }
+    auto put = [&](int part, int n, int k, double v) {
+      for (int q = 0; q < 3; q++) {
+        const uint16_t b = bf16_bits((float)v);
+        out[s * (size_t)stride + (((size_t)part * 3 + q) * NP + n) * KP + k] = b;
+        v -= bf16_value(b);
+      }
+    };
+    for (int i = 0; i < NH; i++) {
+      const bool mid = nd % 2 == 1 && i == NH - 1;  // the middle node, its own mirror
+      for (int c = 0; c < CE; c++) put(0, i, c, M.Pb[(size_t)(2 * c) * nd + i]);
+      for (int c = 0; c < CO; c++) put(1, i, c, mid ? 0.0 : M.Pb[(size_t)(2 * c + 1) * nd + i]);
+      for (int c = 0; c < CE; c++) put(2, c, i, (mid ? 0.5 : 1.0) * M.Wf[(size_t)i * nf + 2 * c]);
+      for (int c = 0; c < CO; c++) put(3, c, i, mid ? 0.0 : M.Wf[(size_t)i * nf + 2 * c + 1]);
+    }
+  }
+  return true;
 }
 
 // LDS limit and work split of the matrix-stationary theta kernel.  A work
@@ -1655,6 +1700,18 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
         h->tfu_lds = flds;
         h->tfu_tpw = h->tmx_tpw;
         h->tfu_nth = TMX_THREADS;
+        // the parity halves, where every set is symmetric and fits (default)
+        std::vector<uint16_t> Q;
+        size_t plds = 0;
+        if (pack_theta_parity(S.mats, Q, h->par_stride) && theta_fused_par_kernel(TMX_THREADS, &plds)) {
+          QD_HIP(hipMalloc(&h->Wpar, Q.size() * 2));
+          QD_HIP(hipMemcpy(h->Wpar, Q.data(), Q.size() * 2, hipMemcpyHostToDevice));
+          for (int nth : {512, 768}) {
+            const void *pf = theta_fused_par_kernel(nth, &plds);
+            if (pf) QD_HIP(hipFuncSetAttribute(pf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+          }
+          h->par_ok = h->tfu_par = true;
+        }
         QD_RC(build_tfu_items(h));
       }
     }
@@ -1868,7 +1925,8 @@ void release(qd_handle *h) {
                   h->bdpsi,    h->fai,    h->dfai,      h->energies, h->eta_sq, h->crp,      h->partial,
                   h->bundle_l0, h->line_off, h->line_lam, h->d_acc_steps, h->mxs[0].head, h->mxs[0].off,
                   h->mxs[0].lam, h->mxs[0].unit, h->mxs[1].head, h->mxs[1].off, h->mxs[1].lam, h->mxs[1].unit,
-                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab, h->crp_part, h->sp_tmp};
+                  h->mx_umat,  h->d_tfu_items, h->stage, h->diag_tab, h->crp_part, h->sp_tmp,
+                  h->Wpar};
   drop_graph(h);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -2056,6 +2114,26 @@ int autotune_handle(qd_handle *h) {
   // fused theta kernel: tile-blocks per work item (items of blocks sharing V
   // are at least one tile per wave, so smaller values matter too)
   if (h->fuse_theta) {
+    // kernel: the dense transforms (512 threads) or the parity halves (512 or
+    // 768 threads), at the starting tile count
+    if (h->par_ok) {
+      const int cpar[3] = {0, 1, 1}, cnth[3] = {TMX_THREADS, 512, 768};
+      float best = 1e30f;
+      int pick = 1;
+      for (int c = 0; c < 3; c++) {
+        h->tfu_par = cpar[c] == 1;
+        h->tfu_nth = cnth[c];
+        QD_RC(build_tfu_items(h));
+        float ms = 0;
+        QD_RC(time_launches(h, [&] { return launch_theta_fused(h, false); }, reps, &ms));
+        if (ms < best) {
+          best = ms;
+          pick = c;
+        }
+      }
+      h->tfu_par = cpar[pick] == 1;
+      h->tfu_nth = cnth[pick];
+    }
     const long long ntiles = (h->n12 + 15) / 16;
     const int base = h->tfu_tpw;
     std::vector<int> cands = {base,     std::max(1, base / 2), std::max(1, base / 4), std::max(1, base / 8),
@@ -2188,7 +2266,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[7] = h->rl_lpb;
   vals[8] = h->rp_mx ? std::max(h->mxs[0].nl, h->mxs[1].nl) : 0;
   vals[9] = h->rp_mx ? std::max(h->mxs[0].gb, h->mxs[1].gb) : 0;
-  vals[10] = h->fuse_theta ? 1 : 0;
+  vals[10] = h->fuse_theta ? (h->tfu_par ? 2 : 1) : 0;
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
   vals[13] = h->fuse_theta ? h->tfu_nth : 0;
   vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
@@ -2550,6 +2628,15 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     drop_graph(h);
     h->mx_chunk = value;
     return QD_OK;
+  }
+  if (what == 3) {
+    if (!h->fuse_theta) return fail(QD_ERR_ARG, "this handle has no fused theta step");
+    if (value < 0 || value > 2) return fail(QD_ERR_ARG, "value must be 0, 1 or 2");
+    if (value > 0 && !h->par_ok) return fail(QD_ERR_ARG, "this handle's theta matrices have no parity halves");
+    drop_graph(h);
+    h->tfu_par = value > 0;
+    h->tfu_nth = value == 2 ? 768 : TMX_THREADS;
+    return build_tfu_items(h);
   }
 
   return fail(QD_ERR_ARG, "unknown selection");

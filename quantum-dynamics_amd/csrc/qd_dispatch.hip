@@ -222,6 +222,19 @@ const void *theta_fused_kernel(int ks, int nth, size_t *lds) {
 #undef QD_TFU
   return nullptr;
 }
+
+// fused kernel on the parity halves (half bases of at most 32 functions)
+const void *theta_fused_par_kernel(int nth, size_t *lds) {
+#define QD_TFP(NTH_)                                                      \
+  if (nth == NTH_) {                                                      \
+    *lds = theta_fused_par_lds<1, NTH_>();                                \
+    return reinterpret_cast<const void *>(&k_theta_fused_par<1, NTH_>);   \
+  }
+  QD_TFP(512)
+  QD_TFP(768)
+#undef QD_TFP
+  return nullptr;
+}
 #endif  // theta part
 
 #if QD_PART == 5 || QD_PART == 6

@@ -1377,6 +1377,231 @@ constexpr size_t theta_fused_lds() {
   return (size_t)2 * 3 * (2 * KS * 16) * (KS * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * (KS * 32 + 1) * 8;
 }
 
+// The fused theta step on the parity halves of the two transforms.  Every
+// theta grid is symmetric, x_{nd-1-i} = -x_i, and FBR function c has parity
+// (-1)^c (P_l(-x) = (-1)^l P_l(x); (-1)^(j-K) for the associated functions of a
+// K block), so with i' = nd - 1 - i and i < NH = ceil(nd / 2):
+//   backward  E_i = sum_{c even} Pb[c][i] F_c,   O_i = sum_{c odd} Pb[c][i] F_c,
+//             DVR_i = E_i + O_i,   DVR_i' = E_i - O_i;
+//   forward   F_c = sum_{i < NH} Wf[i][c] (DVR_i + DVR_i')   (c even),
+//             F_c = sum_{i < NH} Wf[i][c] (DVR_i - DVR_i')   (c odd).
+// An odd grid's middle node is its own mirror: its odd backward row is zero
+// and its even forward row is halved on the host (exact), so the same formulas
+// hold.  Each half product is half as long and half as wide as the full one:
+// half the MFMA work of k_theta_fused and a quarter of its matrix LDS per
+// half.  The host packs the four half matrices of a set as [4][3][NP][KP]
+// (backward even, backward odd, forward even, forward odd; three BF16 pieces;
+// k contiguous) after checking the symmetry numerically.  Everything else --
+// work items, dump, flux rows, V phase, both propagator orders -- is as in
+// k_theta_fused; the per-wave LDS tile holds the sums (columns 0 .. KP - 1)
+// and differences (KP .. 2 KP - 1) for the forward products.
+template <int KSH, int NTH>
+__global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
+  extern __shared__ __align__(16) unsigned char tm_lds[];
+  constexpr int KP = KSH * 32, NT = 2 * KSH, NP = NT * 16, LD = KP + 8, XS = 2 * KP + 1;
+  __bf16 *M = reinterpret_cast<__bf16 *>(tm_lds);  // [4][3][NP][LD]
+  float2 *Xw = reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + (threadIdx.x >> 6) * 16 * XS;
+  const int4 item = P.items[blockIdx.x];
+  const int gb0 = item.x, gnb = item.y, t0 = item.z, t1 = item.w;
+  const BlockDesc bd = P.blocks[gb0];  // the group shares mat, nd, nf and the V columns
+  const int R = P.R, nd = bd.nd, nf = bd.nf;
+  const int NH = (nd + 1) >> 1, CE = (nf + 1) >> 1, CO = nf >> 1;
+  auto rL_of = [&](int b) { return buf_rsrc(P.L + (size_t)P.blocks[gb0 + b].fcol0 * R, (unsigned)nf * R * 8u); };
+  auto rP_of = [&](int b) { return buf_rsrc(P.psi + (size_t)P.blocks[gb0 + b].dcol0 * R, (unsigned)nd * R * 8u); };
+  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(P.expV + P.dcol_vidx[bd.dcol0], (unsigned)nd * R * 8u);
+  {
+    const __bf16 *G = reinterpret_cast<const __bf16 *>(P.Bb) + (size_t)bd.mat * P.mat_stride;
+    constexpr int QV = KP / 8;
+    for (int u = threadIdx.x; u < 4 * 3 * NP * QV; u += blockDim.x) {
+      const int pn = u / QV, kq = u % QV;
+      *reinterpret_cast<bf8v *>(&M[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&G[(size_t)pn * KP + kq * 8]);
+    }
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int rl = lane & 15, kg = lane >> 4;
+  float2 ra[2][8];  // [parity][e]: FBR column 2 (32 s + 8 kg + e) + parity
+  auto fetch = [&](const __amdgpu_buffer_rsrc_t &rd, int tile, int s) {
+    const int row = tile * 16 + rl;
+    const int k0 = s * 32 + kg * 8;
+    const unsigned base = row < R ? (unsigned)row * 8u : TM_OOB;
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int c = 2 * (k0 + e) + q;
+        ra[q][e] = buf_ld_f2(rd, c < nf ? base + (unsigned)(R * c) * 8u : TM_OOB);
+      }
+  };
+  auto split3 = [](const float *x, bf8v *out) {
+    float r[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = x[e];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const __bf16 b = f2bf(r[e]);
+        out[q][e] = b;
+        r[e] -= (float)b;
+      }
+  };
+  int tile = t0 + wave, bi = 0;
+  __amdgpu_buffer_rsrc_t rL = rL_of(0);
+  if (tile < t1) fetch(rL, tile, 0);
+  float2 wv[2][NT][4];  // V phase at columns i and i'
+  while (tile < t1) {
+    const int row = tile * 16 + rl;
+    const bool rin = row < R;
+    const bool lastb = bi + 1 == gnb;
+    const int ntile = lastb ? tile + W : tile, nbi = lastb ? 0 : bi + 1;
+    const __amdgpu_buffer_rsrc_t rLn = rL_of(nbi);
+    if (bi == 0) {
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = nt * 16 + 4 * kg + v;
+          const bool ok = rin && i < NH;
+          wv[0][nt][v] = buf_ld_f2(rV, ok ? (unsigned)(row + R * i) * 8u : TM_OOB);
+          wv[1][nt][v] = buf_ld_f2(rV, ok ? (unsigned)(row + R * (nd - 1 - i)) * 8u : TM_OOB);
+        }
+    }
+    f4v acc[2][2][NT];  // [parity][re, im][n tile]
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < NT; b++) acc[q][a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    // backward: E and O over the even and odd FBR columns
+#pragma unroll 1
+    for (int s = 0; s < KSH; s++) {
+      bf8v bre[2][3], bim[2][3];
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        float re[8], im[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          re[e] = ra[q][e].x;
+          im[e] = ra[q][e].y;
+        }
+        split3(re, bre[q]);
+        split3(im, bim[q]);
+      }
+      if (s + 1 < KSH) fetch(rL, tile, s + 1);
+      else if (ntile < t1) fetch(rLn, ntile, 0);
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+          bf8v a[3];
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+            a[p] = *reinterpret_cast<const bf8v *>(&M[((q * 3 + p) * NP + nt * 16 + rl) * LD + s * 32 + kg * 8]);
+#pragma unroll
+          for (int d = 2; d >= 0; d--)
+#pragma unroll
+            for (int p = 0; p <= d; p++) {
+              acc[q][0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bre[q][d - p], acc[q][0][nt], 0, 0, 0);
+              acc[q][1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bim[q][d - p], acc[q][1][nt], 0, 0, 0);
+            }
+        }
+    }
+    // DVR domain: the pair (i, i'), dump, psi rows, V phase; sums and
+    // differences into the wave's tile
+    float dfac = 1.f;
+    if (P.d1f && rin) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
+    const int rr = row / P.n1;
+    const bool frow = rin && rr >= P.r_lo && rr <= P.r_hi;
+    const __amdgpu_buffer_rsrc_t rP = rP_of(bi);
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = nt * 16 + 4 * kg + v, im = nd - 1 - i;
+        const float2 d1 = make_float2(acc[0][0][nt][v] + acc[1][0][nt][v], acc[0][1][nt][v] + acc[1][1][nt][v]);
+        const float2 d2 = make_float2(acc[0][0][nt][v] - acc[1][0][nt][v], acc[0][1][nt][v] - acc[1][1][nt][v]);
+        const float2 c1 = make_float2(d1.x * dfac, d1.y * dfac), c2 = make_float2(d2.x * dfac, d2.y * dfac);
+        if (frow) {
+          buf_st_f2(P.pre_dump ? d1 : c1, rP, i < NH ? (unsigned)(row + R * i) * 8u : TM_OOB);
+          buf_st_f2(P.pre_dump ? d2 : c2, rP, i < nd / 2 ? (unsigned)(row + R * im) * 8u : TM_OOB);
+        }
+        const float2 w1 = wv[0][nt][v], w2 = wv[1][nt][v];
+        const float2 p1 = make_float2(c1.x * w1.x - c1.y * w1.y, c1.x * w1.y + c1.y * w1.x);
+        const float2 p2 = make_float2(c2.x * w2.x - c2.y * w2.y, c2.x * w2.y + c2.y * w2.x);
+        Xw[rl * XS + i] = make_float2(p1.x + p2.x, p1.y + p2.y);
+        Xw[rl * XS + KP + i] = make_float2(p1.x - p2.x, p1.y - p2.y);
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // forward: even FBR columns from the sums, odd from the differences
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < NT; b++) acc[q][a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s2 = 0; s2 < KSH; s2++) {
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        bf8v fre[3], fim[3];
+        {
+          float re[8], im[8];
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const float2 x = Xw[rl * XS + q * KP + s2 * 32 + kg * 8 + e];
+            re[e] = x.x;
+            im[e] = x.y;
+          }
+          split3(re, fre);
+          split3(im, fim);
+        }
+        if (s2 + 1 == KSH && q == 1) {
+          // reads done before the next tile's writes
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+          bf8v a[3];
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+            a[p] = *reinterpret_cast<const bf8v *>(&M[(((2 + q) * 3 + p) * NP + nt * 16 + rl) * LD + s2 * 32 + kg * 8]);
+#pragma unroll
+          for (int d = 2; d >= 0; d--)
+#pragma unroll
+            for (int p = 0; p <= d; p++) {
+              acc[q][0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[d - p], acc[q][0][nt], 0, 0, 0);
+              acc[q][1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[d - p], acc[q][1][nt], 0, 0, 0);
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int cq = nt * 16 + 4 * kg + v, c = 2 * cq + q;
+          const bool ok = rin && cq < (q ? CO : CE);
+          buf_st_f2(make_float2(acc[q][0][nt][v], acc[q][1][nt][v]), rL, ok ? (unsigned)(row + R * c) * 8u : TM_OOB);
+        }
+    tile = ntile;
+    bi = nbi;
+    rL = rLn;
+  }
+}
+template <int KSH, int NTH>
+constexpr size_t theta_fused_par_lds() {
+  return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * (2 * KSH * 32 + 1) * 8;
+}
+
 template <int KS, int NT>
 constexpr size_t theta_mx_lds() {
   return (size_t)3 * NT * 16 * (KS * 32 + 8) * 2;

@@ -49,7 +49,7 @@ def test_mixed_fused_theta_with_diagnostics(qd, oracle):
     p = qd.problem.config(0)
     print("case: mixed diagnostics every 5 steps, 40 steps", flush=True)
     with qd.Propagator(p.copy(), precision="mixed") as a:
-        assert a.tuning()["fused_theta"] == 1
+        assert a.tuning()["fused_theta"] >= 1
         a.propagate(40)
         psi_a, crp_a = a.psi(), a.crp()
     with qd.Propagator(p.copy(), precision="mixed", diagnostics_every=5) as b:
@@ -74,14 +74,14 @@ def test_cuda_order_fused_theta_with_diagnostics(qd):
     p = qd.problem.config(0)
     print("case: mixed cuda-order diagnostics every 5 steps, 40 steps", flush=True)
     with qd.Propagator(p.copy(), precision="mixed", variant="cuda") as a:
-        assert a.tuning()["fused_theta"] == 1
+        assert a.tuning()["fused_theta"] >= 1
         a.propagate(40)
         psi_a, crp_a = a.psi(), a.crp()
     out = {}
     for fuse in (True, False):
         with qd.Propagator(p.copy(), precision="mixed", variant="cuda", diagnostics_every=5, crp_every=7,
                            fuse_theta=fuse) as b:
-            assert b.tuning()["fused_theta"] == (1 if fuse else 0)
+            assert (b.tuning()["fused_theta"] >= 1) == bool(fuse)
             seen = []
             b.set_callback(lambda steps, crp: seen.append(steps) or False)
             b.propagate(23)

@@ -77,7 +77,11 @@ def test_jk_norm_conserved_without_absorbers(qd, jkcase, prec):
     with qd.propagator.JKPropagator(jk2, precision=prec) as prop:
         prop.propagate(60)
         n1 = prop.norm()
-    assert abs(n1 - n0) < (1e-12 if prec == "fp64" else 1e-6) * n0
+    # mixed: the per-step bound of test_mixed_norm_drift, 5e-8; on these 5..8-point
+    # theta bases the FP32 rounding alone moves the norm by about 1e-8 per step
+    # (-0.5e-6 .. -1.1e-6 after 60 steps, -2e-6 .. -3e-6 after 240, for every
+    # theta kernel and FFT table form measured), with no trend beyond that
+    assert abs(n1 - n0) < (1e-12 if prec == "fp64" else 60 * 5e-8) * n0
 
 
 def test_jk_prefetch_mixing_mixed_within_1e4_of_fp64(qd):

@@ -153,7 +153,8 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
     between: psi (rebuilt from the FBR state, or in the CUDA propagator's
     order written by the step before the read) and P(E) agree to the rounding
     level of the mixed path, and are no further from the FP64 path than the
-    unfused ones."""
+    unfused ones.  Each fused kernel is checked: the dense transforms, and
+    their parity halves with 512 and 768 threads per work group."""
     pb = qd.problem
     print("case: fused theta vs unfused,", kind, variant, flush=True)
     if kind == "single":
@@ -168,29 +169,35 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
     else:
         mk64 = lambda: qd.propagator.JKPropagator(jk, precision="fp64", variant=variant)
     out = {}
-    for fuse in (False, True, "fp64"):
-        with (mk64() if fuse == "fp64" else mk(fuse)) as prop:
-            if fuse != "fp64":
-                assert prop.tuning()["fused_theta"] == (1 if fuse else 0)
+    for fuse in (False, 0, 1, 2, "fp64"):
+        with (mk64() if fuse == "fp64" else mk(fuse is not False)) as prop:
+            if fuse is False:
+                assert prop.tuning()["fused_theta"] == 0
+            elif fuse != "fp64":
+                assert prop.tuning()["fused_theta"] == 2  # parity halves by default
+                prop.debug_select(3, fuse)
+                assert prop.tuning()["fused_theta"] == (1 if fuse == 0 else 2)
             prop.propagate(25)
             n_mid = prop.norm()
             prop.propagate(35)
             out[fuse] = (prop.psi(), prop.crp(), n_mid, prop.norm())
     psi_u, crp_u, nm_u, n_u = out[False]
-    psi_f, crp_f, nm_f, n_f = out[True]
     psi_d, crp_d = out["fp64"][0], out["fp64"][1]
-    scale = np.max(np.abs(psi_d))
-    d_fu, d_ud, d_fd = (np.max(np.abs(a - b)) / scale for a, b in ((psi_f, psi_u), (psi_u, psi_d), (psi_f, psi_d)))
-    print("psi: |fused - unfused| %.2e, |unfused - fp64| %.2e, |fused - fp64| %.2e" % (d_fu, d_ud, d_fd))
-    assert d_fu <= 1e-5 and d_fd <= 2 * d_ud + 1e-6
-    cs = max(np.max(np.abs(crp_d)), 1e-2)
-    assert np.max(np.abs(crp_f - crp_u)) <= 1e-6 * cs
-    assert np.max(np.abs(crp_f - crp_d)) <= 2 * np.max(np.abs(crp_u - crp_d)) + 1e-7 * cs
     nm_d, n_d = out["fp64"][2], out["fp64"][3]
-    print("norm after 25 / 60 steps: fp64 %.9f %.9f, unfused %.9f %.9f, fused %.9f %.9f"
-          % (nm_d, n_d, nm_u, n_u, nm_f, n_f))
-    assert abs(nm_f - nm_d) <= 2 * abs(nm_u - nm_d) + 1e-6 * nm_d
-    assert abs(n_f - n_d) <= 2 * abs(n_u - n_d) + 1e-6 * n_d
+    scale = np.max(np.abs(psi_d))
+    cs = max(np.max(np.abs(crp_d)), 1e-2)
+    for kernel in (0, 1, 2):
+        psi_f, crp_f, nm_f, n_f = out[kernel]
+        d_fu, d_ud, d_fd = (np.max(np.abs(a - b)) / scale for a, b in ((psi_f, psi_u), (psi_u, psi_d), (psi_f, psi_d)))
+        print("fused kernel %d psi: |fused - unfused| %.2e, |unfused - fp64| %.2e, |fused - fp64| %.2e"
+              % (kernel, d_fu, d_ud, d_fd))
+        assert d_fu <= 1e-5 and d_fd <= 2 * d_ud + 1e-6
+        assert np.max(np.abs(crp_f - crp_u)) <= 1e-6 * cs
+        assert np.max(np.abs(crp_f - crp_d)) <= 2 * np.max(np.abs(crp_u - crp_d)) + 1e-7 * cs
+        print("norm after 25 / 60 steps: fp64 %.9f %.9f, unfused %.9f %.9f, fused %.9f %.9f"
+              % (nm_d, n_d, nm_u, n_u, nm_f, n_f))
+        assert abs(nm_f - nm_d) <= 2 * abs(nm_u - nm_d) + 1e-6 * nm_d
+        assert abs(n_f - n_d) <= 2 * abs(n_u - n_d) + 1e-6 * n_d
 
 
 @pytest.fixture(scope="module")
