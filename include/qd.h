@@ -254,9 +254,10 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * what 1: its consecutive bundles per turn of a work group (1..64, default 8).
  * what 2: the split mixing R pass on (value 1) or off (value 0); QD_ERR_ARG
  * when the handle does not have it, or has no other R pass for its groups.
- * what 3: the fused theta step on the dense transforms (value 0) or on their
- * parity halves with 512 (1) or 768 (2) threads per work group; QD_ERR_ARG
- * when the handle has no fused step or no parity halves. */
+ * what 3: the fused theta step on the dense transforms (value 0; bases of at
+ * most 64) or on their parity halves (1: 512 threads per work group, 256 for
+ * bases of 65..128; 2: 768 threads, bases of at most 64); QD_ERR_ARG when the
+ * handle does not have that kernel. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

@@ -244,7 +244,8 @@ struct qd_handle {
   int fuse_opt = 0, tfu_tpw = 1, tfu_nth = TMX_THREADS;  // tile-blocks per item, threads per work group
   size_t tfu_lds = 0;
   // the fused step on the parity halves of the theta transforms (k_theta_fused_par)
-  bool tfu_par = false, par_ok = false;
+  bool tfu_par = false, par_ok = false, dense_ok = false;  // kernel in use; which kernels fit
+  int par_ksh = 1;                                         // k-steps of 32 per half basis
   void *Wpar = nullptr;
   long long par_stride = 0;
   // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
@@ -605,8 +606,8 @@ int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   F.pre_dump = h->variant == QD_VARIANT_CUDA ? 1 : 0;
   F.items = h->d_tfu_items;
   size_t lds = 0;
-  const void *fn =
-      h->tfu_par ? theta_fused_par_kernel(h->tfu_nth, &lds) : theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
+  const void *fn = h->tfu_par ? theta_fused_par_kernel(h->par_ksh, h->tfu_nth, &lds)
+                              : theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
   if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
   if (h->tfu_nitems < 1) return fail(QD_ERR_ARG, "fused theta work items not set up");
   dim3 grid((unsigned)h->tfu_nitems, 1, 1);
@@ -1015,12 +1016,12 @@ int put_lift_tables(qd_handle *h, int n, const std::vector<double> &kin_ang, con
 }
 
 // Parity halves of the theta matrices for k_theta_fused_par: per set
-// [4][3][32][32] BF16 pieces (backward even, backward odd, forward even,
-// forward odd; k contiguous).  False when a half has more than 32 functions or
-// points, or a set is not symmetric under x -> -x with FBR parity (-1)^c to
-// 1e-10 of its largest entry (the kernel relies on it).
-bool pack_theta_parity(const std::vector<MatSet> &mats, std::vector<uint16_t> &out, long long &stride) {
-  constexpr int NP = 32, KP = 32;
+// [4][3][32 ksh][32 ksh] BF16 pieces (backward even, backward odd, forward
+// even, forward odd; k contiguous).  False when a half has more than 32 ksh
+// functions or points, or a set is not symmetric under x -> -x with FBR parity
+// (-1)^c to 1e-10 of its largest entry (the kernel relies on it).
+bool pack_theta_parity(const std::vector<MatSet> &mats, int ksh, std::vector<uint16_t> &out, long long &stride) {
+  const int NP = 32 * ksh, KP = 32 * ksh;
   stride = 4LL * 3 * NP * KP;
   out.assign(mats.size() * (size_t)stride, 0);
   for (size_t s = 0; s < mats.size(); s++) {
@@ -1687,31 +1688,35 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
     QD_HIP(hipMemcpy(db, B.data(), B.size() * 2, hipMemcpyHostToDevice));
     h->Wf = df;
     h->Wb = db;
-    // fused theta step: either propagator order, three pieces, both matrices in LDS
-    size_t flds = 0;
-    if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->nsplit == 3 &&
-        h->tmx_ks >= 1 && h->tmx_ks <= 2 && theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds)) {
-      const void *fn = theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds);
-      QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
-      int nbk = 0;
-      QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, TMX_THREADS, flds));
-      if (nbk >= 1) {
+    // fused theta step, either propagator order, three pieces, both matrices
+    // in LDS: the dense transforms (bases up to 64) or their parity halves
+    // (halves up to 64 functions: bases up to 128), the halves by default
+    if (h->fuse_opt == 0 && h->pipe == QD_PIPE_FUSED && h->nsplit == 3 && h->tmx_ks >= 1) {
+      auto fits = [&](const void *fn, int nth, size_t lds) -> bool {
+        if (!fn) return false;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return false;
+        int nbk = 0;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, nth, lds) == hipSuccess && nbk >= 1;
+      };
+      size_t flds = 0;
+      h->dense_ok = h->tmx_ks <= 2 && fits(theta_fused_kernel(h->tmx_ks, TMX_THREADS, &flds), TMX_THREADS, flds);
+      const int ksh = ((maxn + 1) / 2 + 31) / 32;
+      const int pnth = ksh == 1 ? TMX_THREADS : 256;
+      size_t plds = 0;
+      std::vector<uint16_t> Q;
+      if (ksh <= 2 && fits(theta_fused_par_kernel(ksh, pnth, &plds), pnth, plds) &&
+          pack_theta_parity(S.mats, ksh, Q, h->par_stride)) {
+        QD_HIP(hipMalloc(&h->Wpar, Q.size() * 2));
+        QD_HIP(hipMemcpy(h->Wpar, Q.data(), Q.size() * 2, hipMemcpyHostToDevice));
+        if (ksh == 1) (void)fits(theta_fused_par_kernel(1, 768, &plds), 768, plds);
+        h->par_ok = true;
+        h->par_ksh = ksh;
+      }
+      if (h->dense_ok || h->par_ok) {
         h->fuse_theta = true;
-        h->tfu_lds = flds;
+        h->tfu_par = h->par_ok;
+        h->tfu_nth = h->par_ok ? pnth : TMX_THREADS;
         h->tfu_tpw = h->tmx_tpw;
-        h->tfu_nth = TMX_THREADS;
-        // the parity halves, where every set is symmetric and fits (default)
-        std::vector<uint16_t> Q;
-        size_t plds = 0;
-        if (pack_theta_parity(S.mats, Q, h->par_stride) && theta_fused_par_kernel(TMX_THREADS, &plds)) {
-          QD_HIP(hipMalloc(&h->Wpar, Q.size() * 2));
-          QD_HIP(hipMemcpy(h->Wpar, Q.data(), Q.size() * 2, hipMemcpyHostToDevice));
-          for (int nth : {512, 768}) {
-            const void *pf = theta_fused_par_kernel(nth, &plds);
-            if (pf) QD_HIP(hipFuncSetAttribute(pf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
-          }
-          h->par_ok = h->tfu_par = true;
-        }
         QD_RC(build_tfu_items(h));
       }
     }
@@ -2116,11 +2121,12 @@ int autotune_handle(qd_handle *h) {
   if (h->fuse_theta) {
     // kernel: the dense transforms (512 threads) or the parity halves (512 or
     // 768 threads), at the starting tile count
-    if (h->par_ok) {
+    if (h->par_ok && (h->dense_ok || h->par_ksh == 1)) {
       const int cpar[3] = {0, 1, 1}, cnth[3] = {TMX_THREADS, 512, 768};
       float best = 1e30f;
       int pick = 1;
       for (int c = 0; c < 3; c++) {
+        if ((cpar[c] == 0 && !h->dense_ok) || (cpar[c] == 1 && h->par_ksh != 1)) continue;
         h->tfu_par = cpar[c] == 1;
         h->tfu_nth = cnth[c];
         QD_RC(build_tfu_items(h));
@@ -2633,9 +2639,11 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     if (!h->fuse_theta) return fail(QD_ERR_ARG, "this handle has no fused theta step");
     if (value < 0 || value > 2) return fail(QD_ERR_ARG, "value must be 0, 1 or 2");
     if (value > 0 && !h->par_ok) return fail(QD_ERR_ARG, "this handle's theta matrices have no parity halves");
+    if (value == 0 && !h->dense_ok) return fail(QD_ERR_ARG, "the dense fused kernel does not fit this basis");
+    if (value == 2 && h->par_ksh != 1) return fail(QD_ERR_ARG, "768 threads only for half bases of at most 32");
     drop_graph(h);
     h->tfu_par = value > 0;
-    h->tfu_nth = value == 2 ? 768 : TMX_THREADS;
+    h->tfu_nth = value == 2 ? 768 : value == 1 && h->par_ksh == 2 ? 256 : TMX_THREADS;
     return build_tfu_items(h);
   }
 

@@ -223,15 +223,18 @@ const void *theta_fused_kernel(int ks, int nth, size_t *lds) {
   return nullptr;
 }
 
-// fused kernel on the parity halves (half bases of at most 32 functions)
-const void *theta_fused_par_kernel(int nth, size_t *lds) {
-#define QD_TFP(NTH_)                                                      \
-  if (nth == NTH_) {                                                      \
-    *lds = theta_fused_par_lds<1, NTH_>();                                \
-    return reinterpret_cast<const void *>(&k_theta_fused_par<1, NTH_>);   \
+// fused kernel on the parity halves: half bases of at most 32 (ksh 1) or 64
+// (ksh 2, the tile one half at a time) functions
+const void *theta_fused_par_kernel(int ksh, int nth, size_t *lds) {
+#define QD_TFP(KSH_, NTH_)                                                  \
+  if (ksh == KSH_ && nth == NTH_) {                                         \
+    *lds = theta_fused_par_lds<KSH_, NTH_>();                               \
+    return reinterpret_cast<const void *>(&k_theta_fused_par<KSH_, NTH_>);  \
   }
-  QD_TFP(512)
-  QD_TFP(768)
+  QD_TFP(1, 512)
+  QD_TFP(1, 768)
+  QD_TFP(2, 256)
+  QD_TFP(2, 384)
 #undef QD_TFP
   return nullptr;
 }

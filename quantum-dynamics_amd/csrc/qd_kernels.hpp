@@ -1394,11 +1394,16 @@ constexpr size_t theta_fused_lds() {
 // k contiguous) after checking the symmetry numerically.  Everything else --
 // work items, dump, flux rows, V phase, both propagator orders -- is as in
 // k_theta_fused; the per-wave LDS tile holds the sums (columns 0 .. KP - 1)
-// and differences (KP .. 2 KP - 1) for the forward products.
+// and differences (KP .. 2 KP - 1) for the forward products -- or, for half
+// bases of 33..64 (KSH = 2, whose four half matrices take 110.6 KB), the sums
+// and then the differences in one KP-column tile, so that the kernel fits a
+// CU's LDS with four or six waves.
 template <int KSH, int NTH>
 __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
   extern __shared__ __align__(16) unsigned char tm_lds[];
-  constexpr int KP = KSH * 32, NT = 2 * KSH, NP = NT * 16, LD = KP + 8, XS = 2 * KP + 1;
+  constexpr bool TWOPASS = KSH > 1;
+  constexpr int KP = KSH * 32, NT = 2 * KSH, NP = NT * 16, LD = KP + 8, XS = (TWOPASS ? 1 : 2) * KP + 1;
+  constexpr int QOFF = TWOPASS ? 0 : KP;  // tile column of the differences
   __bf16 *M = reinterpret_cast<__bf16 *>(tm_lds);  // [4][3][NP][LD]
   float2 *Xw = reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + (threadIdx.x >> 6) * 16 * XS;
   const int4 item = P.items[blockIdx.x];
@@ -1516,6 +1521,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
     const int rr = row / P.n1;
     const bool frow = rin && rr >= P.r_lo && rr <= P.r_hi;
     const __amdgpu_buffer_rsrc_t rP = rP_of(bi);
+    float2 pp[2][NT][4];  // the pair's values after dump and V phase
 #pragma unroll
     for (int nt = 0; nt < NT; nt++)
 #pragma unroll
@@ -1529,14 +1535,29 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
           buf_st_f2(P.pre_dump ? d2 : c2, rP, i < nd / 2 ? (unsigned)(row + R * im) * 8u : TM_OOB);
         }
         const float2 w1 = wv[0][nt][v], w2 = wv[1][nt][v];
-        const float2 p1 = make_float2(c1.x * w1.x - c1.y * w1.y, c1.x * w1.y + c1.y * w1.x);
-        const float2 p2 = make_float2(c2.x * w2.x - c2.y * w2.y, c2.x * w2.y + c2.y * w2.x);
-        Xw[rl * XS + i] = make_float2(p1.x + p2.x, p1.y + p2.y);
-        Xw[rl * XS + KP + i] = make_float2(p1.x - p2.x, p1.y - p2.y);
+        pp[0][nt][v] = make_float2(c1.x * w1.x - c1.y * w1.y, c1.x * w1.y + c1.y * w1.x);
+        pp[1][nt][v] = make_float2(c2.x * w2.x - c2.y * w2.y, c2.x * w2.y + c2.y * w2.x);
       }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // sums (q = 0) and differences (q = 1) into the tile: both at once, or
+    // one at a time when the tile holds one half
+    auto put_tile = [&](int q) {
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = nt * 16 + 4 * kg + v;
+          const float2 p1 = pp[0][nt][v], p2 = pp[1][nt][v];
+          Xw[rl * XS + q * QOFF + i] = q == 0 ? make_float2(p1.x + p2.x, p1.y + p2.y) : make_float2(p1.x - p2.x, p1.y - p2.y);
+        }
+    };
+    auto wave_sync = [] {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    put_tile(0);
+    if constexpr (!TWOPASS) put_tile(1);
+    wave_sync();
     // forward: even FBR columns from the sums, odd from the differences
 #pragma unroll
     for (int q = 0; q < 2; q++)
@@ -1545,27 +1566,27 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
 #pragma unroll
         for (int b = 0; b < NT; b++) acc[q][a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s2 = 0; s2 < KSH; s2++) {
+    for (int q = 0; q < 2; q++) {
+      if (TWOPASS && q == 1) {
+        put_tile(1);
+        wave_sync();
+      }
 #pragma unroll
-      for (int q = 0; q < 2; q++) {
+      for (int s2 = 0; s2 < KSH; s2++) {
         bf8v fre[3], fim[3];
         {
           float re[8], im[8];
 #pragma unroll
           for (int e = 0; e < 8; e++) {
-            const float2 x = Xw[rl * XS + q * KP + s2 * 32 + kg * 8 + e];
+            const float2 x = Xw[rl * XS + q * QOFF + s2 * 32 + kg * 8 + e];
             re[e] = x.x;
             im[e] = x.y;
           }
           split3(re, fre);
           split3(im, fim);
         }
-        if (s2 + 1 == KSH && q == 1) {
-          // reads done before the next tile's writes
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+        // reads done before the next writes (the differences, or the next tile)
+        if (s2 + 1 == KSH && (TWOPASS || q == 1)) wave_sync();
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
           bf8v a[3];
@@ -1599,7 +1620,8 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
 }
 template <int KSH, int NTH>
 constexpr size_t theta_fused_par_lds() {
-  return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * (2 * KSH * 32 + 1) * 8;
+  return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 +
+         (size_t)(NTH / 64) * 16 * ((KSH > 1 ? 1 : 2) * KSH * 32 + 1) * 8;
 }
 
 template <int KS, int NT>

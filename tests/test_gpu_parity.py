@@ -145,7 +145,7 @@ def test_autotuned_handle_matches(qd, oracle, hh2_small, prec):
 
 
 @pytest.mark.parametrize("variant", ["cpu", "cuda"])
-@pytest.mark.parametrize("kind", ["single", "batched"])
+@pytest.mark.parametrize("kind", ["single", "batched", "wide"])
 def test_fused_theta_step_matches_unfused(qd, kind, variant):
     """The fused theta step (theta-backward, dump, flux rows, V phase and the
     next theta-forward in one kernel; the state between steps is the FBR
@@ -154,22 +154,25 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
     order written by the step before the read) and P(E) agree to the rounding
     level of the mixed path, and are no further from the FP64 path than the
     unfused ones.  Each fused kernel is checked: the dense transforms, and
-    their parity halves with 512 and 768 threads per work group."""
+    their parity halves with 512 and 768 threads per work group; on a
+    101-point theta grid (half bases of 51 and an odd middle node) the only
+    fused kernel, the parity halves with two k-steps per half."""
     pb = qd.problem
     print("case: fused theta vs unfused,", kind, variant, flush=True)
-    if kind == "single":
-        prob = pb.config(0, steps=60)
+    kernels = (1,) if kind == "wide" else (0, 1, 2)
+    if kind in ("single", "wide"):
+        prob = pb.config(0, steps=60) if kind == "single" else pb.config(0, ntheta=101, steps=60)
         mk = lambda fuse: qd.Propagator(prob.copy(), precision="mixed", variant=variant, fuse_theta=fuse)
     else:
         base = pb.config(0, nR=64, nr=32, ntheta=12, steps=60, e0_ev=0.6)
         jk = pb.make_jk_problem(base, pb.H_H2, [0, 3, 7], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
         mk = lambda fuse: qd.propagator.JKPropagator(jk, precision="mixed", variant=variant, fuse_theta=fuse)
-    if kind == "single":
+    if kind in ("single", "wide"):
         mk64 = lambda: qd.Propagator(prob.copy(), precision="fp64", variant=variant)
     else:
         mk64 = lambda: qd.propagator.JKPropagator(jk, precision="fp64", variant=variant)
     out = {}
-    for fuse in (False, 0, 1, 2, "fp64"):
+    for fuse in (False,) + kernels + ("fp64",):
         with (mk64() if fuse == "fp64" else mk(fuse is not False)) as prop:
             if fuse is False:
                 assert prop.tuning()["fused_theta"] == 0
@@ -186,7 +189,7 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
     nm_d, n_d = out["fp64"][2], out["fp64"][3]
     scale = np.max(np.abs(psi_d))
     cs = max(np.max(np.abs(crp_d)), 1e-2)
-    for kernel in (0, 1, 2):
+    for kernel in kernels:
         psi_f, crp_f, nm_f, n_f = out[kernel]
         d_fu, d_ud, d_fd = (np.max(np.abs(a - b)) / scale for a, b in ((psi_f, psi_u), (psi_u, psi_d), (psi_f, psi_d)))
         print("fused kernel %d psi: |fused - unfused| %.2e, |unfused - fp64| %.2e, |fused - fp64| %.2e"
