@@ -257,7 +257,9 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * what 3: the fused theta step on the dense transforms (value 0; bases of at
  * most 64) or on their parity halves (1: 512 threads per work group, 256 for
  * bases of 65..128; 2: 768 threads, bases of at most 64); QD_ERR_ARG when the
- * handle does not have that kernel. */
+ * handle does not have that kernel.
+ * what 4: threads per work group of the register-FFT r pass, 256 or 512
+ * (batches of twice the consecutive R points); QD_ERR_ARG when it does not fit. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

@@ -222,6 +222,8 @@ struct qd_handle {
   bool rp_rf = false, rl_rf = false;
   int rp_lw = 0;
   long long rp_slots = 1, rl_slots = 1;  // resident work groups of the persistent passes
+  int rl_nth = 256;                       // threads per work group of the register-FFT r pass (256 or 512)
+  long long rl_slots2 = 0;                // resident 512-thread work groups of it (0: does not fit)
   int *bundle_l0 = nullptr, *line_lam = nullptr;  // mixing R pass: per-bundle line tables
   // FP32 mixing R pass with prefetch (k_rpass_mx): bundle records
   bool rp_mx = false, rp_legacy = true;  // legacy: the LDS-staged and Stockham R passes are set up
@@ -498,10 +500,11 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   if (h->rl_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
     size_t lds = 0;
     int lpb = 0;
-    const void *fn = rline_rf_kernel<T>(h->n2, &lds, &lpb);
+    const void *fn = rline_rf_kernel<T>(h->n2, h->rl_nth, &lds, &lpb);
     void *args[] = {&a};
-    const unsigned nblk = (unsigned)std::min(h->rl_slots, ((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
-    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, h->stream));
+    const long long slots = h->rl_nth == 512 ? h->rl_slots2 : h->rl_slots;
+    const unsigned nblk = (unsigned)std::min(slots, ((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
+    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->rl_nth), args, lds, h->stream));
     QD_HIP(hipGetLastError());
     return QD_OK;
   }
@@ -1354,7 +1357,8 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   if (rf_length(n2) && n1 % (256 / rf_n1(n2)) == 0) {
     size_t lds = 0;
     int lpb = 0;
-    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, &lds, &lpb) : rline_rf_kernel<float>(n2, &lds, &lpb);
+    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, 256, &lds, &lpb)
+                                             : rline_rf_kernel<float>(n2, 256, &lds, &lpb);
     h->rl_rf = fn && lds <= 160 * 1024;
   }
   // theta matrices: padded shapes shared by all sets
@@ -1408,11 +1412,22 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
   if (h->rl_rf) {
     size_t lds = 0;
     int lpb = 0;
-    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, &lds, &lpb) : rline_rf_kernel<float>(n2, &lds, &lpb);
+    const void *fn = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, 256, &lds, &lpb)
+                                             : rline_rf_kernel<float>(n2, 256, &lds, &lpb);
     QD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int nb = 0;
     QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds));
     h->rl_slots = std::max(1, nb) * std::max(1, ncu);
+    // the 512-thread variant (batches of twice the consecutive R points), where it fits
+    const void *fn2 = h->prec == QD_PREC_FP64 ? rline_rf_kernel<double>(n2, 512, &lds, &lpb)
+                                              : rline_rf_kernel<float>(n2, 512, &lds, &lpb);
+    h->rl_slots2 = 0;
+    if (fn2 && lds <= 160 * 1024 && n1 % lpb == 0) {
+      QD_HIP(hipFuncSetAttribute(fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      int nb2 = 0;
+      QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, fn2, 512, lds));
+      if (nb2 >= 1) h->rl_slots2 = (long long)nb2 * std::max(1, ncu);
+    }
   }
   if (sp_cand && !units.empty()) {
     size_t lds = 0;
@@ -2219,6 +2234,18 @@ int autotune_handle(qd_handle *h) {
     }
   }
   QD_RC(pick_pass(&h->rl_rf, false));
+  // register-FFT r pass: 256 or 512 threads per work group
+  if (h->rl_rf && h->rl_slots2 > 0) {
+    float t256 = 0, t512 = 0;
+    auto run = [&] {
+      return f64 ? launch_rline<double>(h, full, (double2 *)lpsi) : launch_rline<float>(h, full, (float2 *)lpsi);
+    };
+    h->rl_nth = 256;
+    QD_RC(time_launches(h, run, reps, &t256));
+    h->rl_nth = 512;
+    QD_RC(time_launches(h, run, reps, &t512));
+    h->rl_nth = t512 < t256 ? 512 : 256;
+  }
   h->tuned = true;
   return QD_OK;
 }
@@ -2269,7 +2296,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[4] = h->rl_rf ? 1 : 0;
   vals[5] = h->rp_lw;
   vals[6] = h->rp_ml;
-  vals[7] = h->rl_lpb;
+  vals[7] = h->rl_rf ? h->rl_nth / rf_n1(h->n2) : h->rl_lpb;
   vals[8] = h->rp_mx ? std::max(h->mxs[0].nl, h->mxs[1].nl) : 0;
   vals[9] = h->rp_mx ? std::max(h->mxs[0].gb, h->mxs[1].gb) : 0;
   vals[10] = h->fuse_theta ? (h->tfu_par ? 2 : 1) : 0;
@@ -2633,6 +2660,14 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     if (value < 1 || value > 64) return fail(QD_ERR_ARG, "chunk must be 1..64");
     drop_graph(h);
     h->mx_chunk = value;
+    return QD_OK;
+  }
+  if (what == 4) {
+    if (value != 256 && value != 512) return fail(QD_ERR_ARG, "value must be 256 or 512");
+    if (!h->rl_rf) return fail(QD_ERR_ARG, "this handle has no register-FFT r pass");
+    if (value == 512 && h->rl_slots2 == 0) return fail(QD_ERR_ARG, "the 512-thread r pass does not fit this handle");
+    drop_graph(h);
+    h->rl_nth = value;
     return QD_OK;
   }
   if (what == 3) {

@@ -113,14 +113,22 @@ const void *rpass_rf_kernel(int n, bool mix, int lw, size_t *lds) {
       return nullptr;
   }
 }
+// nth: threads per work group, 256 or 512 (batches of nth / N1 consecutive R points)
 template <typename T>
-const void *rline_rf_kernel(int n, size_t *lds, int *lpb) {
+const void *rline_rf_kernel(int n, int nth, size_t *lds, int *lpb) {
   switch (n) {
-#define QD_CASE(N, N1)                             \
-  case N:                                          \
-    *lds = rline_rf_lds<T, N1, N / N1>();          \
-    *lpb = 256 / N1;                               \
-    return (const void *)k_rline_rf<T, N1, N / N1>;
+#define QD_CASE(N, N1)                                                   \
+  case N:                                                                \
+    *lpb = nth / N1;                                                     \
+    if (nth == 256) {                                                    \
+      *lds = rline_rf_lds<T, N1, N / N1, 256>();                         \
+      return (const void *)k_rline_rf<T, N1, N / N1, 256>;               \
+    }                                                                    \
+    if (nth == 512) {                                                    \
+      *lds = rline_rf_lds<T, N1, N / N1, 512>();                         \
+      return (const void *)k_rline_rf<T, N1, N / N1, 512>;               \
+    }                                                                    \
+    return nullptr;
     QD_RF_LENGTHS(QD_CASE)
 #undef QD_CASE
     default:
@@ -129,10 +137,10 @@ const void *rline_rf_kernel(int n, size_t *lds, int *lpb) {
 }
 #if QD_PART == 2
 template const void *rpass_rf_kernel<float>(int, bool, int, size_t *);
-template const void *rline_rf_kernel<float>(int, size_t *, int *);
+template const void *rline_rf_kernel<float>(int, int, size_t *, int *);
 #else
 template const void *rpass_rf_kernel<double>(int, bool, int, size_t *);
-template const void *rline_rf_kernel<double>(int, size_t *, int *);
+template const void *rline_rf_kernel<double>(int, int, size_t *, int *);
 #endif
 #endif  // register-FFT parts
 
@@ -234,7 +242,6 @@ const void *theta_fused_par_kernel(int ksh, int nth, size_t *lds) {
   QD_TFP(1, 512)
   QD_TFP(1, 768)
   QD_TFP(2, 256)
-  QD_TFP(2, 384)
 #undef QD_TFP
   return nullptr;
 }

@@ -27,7 +27,7 @@ template <typename T>
 const void *rpass_rf_mix_kernel_long(int n, int lw, size_t *lds);
 // register-FFT r pass: *lds = dynamic LDS bytes, *lpb = lines per work group
 template <typename T>
-const void *rline_rf_kernel(int n, size_t *lds, int *lpb);
+const void *rline_rf_kernel(int n, int nth, size_t *lds, int *lpb);
 // FP32 mixing R pass with prefetch (k_rpass_mx) for line length n, nl lines
 // per bundle and unit sizes up to gb, sized for one or (two) two resident work
 // groups per CU; *lds = its dynamic LDS bytes.  nullptr when no instantiation

@@ -1395,15 +1395,13 @@ constexpr size_t theta_fused_lds() {
 // work items, dump, flux rows, V phase, both propagator orders -- is as in
 // k_theta_fused; the per-wave LDS tile holds the sums (columns 0 .. KP - 1)
 // and differences (KP .. 2 KP - 1) for the forward products -- or, for half
-// bases of 33..64 (KSH = 2, whose four half matrices take 110.6 KB), the sums
-// and then the differences in one KP-column tile, so that the kernel fits a
-// CU's LDS with four or six waves.
+// bases of 33..64 (KSH = 2, whose four half matrices take 110.6 KB), the 32
+// columns of one forward k-step at a time (four rounds), so that the kernel
+// fits a CU's LDS with eight waves.
 template <int KSH, int NTH>
 __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
   extern __shared__ __align__(16) unsigned char tm_lds[];
-  constexpr bool TWOPASS = KSH > 1;
-  constexpr int KP = KSH * 32, NT = 2 * KSH, NP = NT * 16, LD = KP + 8, XS = (TWOPASS ? 1 : 2) * KP + 1;
-  constexpr int QOFF = TWOPASS ? 0 : KP;  // tile column of the differences
+  constexpr int KP = KSH * 32, NT = 2 * KSH, NP = NT * 16, LD = KP + 8, XS = (KSH > 1 ? 32 : 2 * KP) + 1;
   __bf16 *M = reinterpret_cast<__bf16 *>(tm_lds);  // [4][3][NP][LD]
   float2 *Xw = reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + (threadIdx.x >> 6) * 16 * XS;
   const int4 item = P.items[blockIdx.x];
@@ -1538,70 +1536,81 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
         pp[0][nt][v] = make_float2(c1.x * w1.x - c1.y * w1.y, c1.x * w1.y + c1.y * w1.x);
         pp[1][nt][v] = make_float2(c2.x * w2.x - c2.y * w2.y, c2.x * w2.y + c2.y * w2.x);
       }
-    // sums (q = 0) and differences (q = 1) into the tile: both at once, or
-    // one at a time when the tile holds one half
-    auto put_tile = [&](int q) {
-#pragma unroll
-      for (int nt = 0; nt < NT; nt++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int i = nt * 16 + 4 * kg + v;
-          const float2 p1 = pp[0][nt][v], p2 = pp[1][nt][v];
-          Xw[rl * XS + q * QOFF + i] = q == 0 ? make_float2(p1.x + p2.x, p1.y + p2.y) : make_float2(p1.x - p2.x, p1.y - p2.y);
-        }
-    };
     auto wave_sync = [] {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    put_tile(0);
-    if constexpr (!TWOPASS) put_tile(1);
-    wave_sync();
-    // forward: even FBR columns from the sums, odd from the differences
+    auto sum_or_diff = [&](int q, int nt, int v) {
+      const float2 p1 = pp[0][nt][v], p2 = pp[1][nt][v];
+      return q == 0 ? make_float2(p1.x + p2.x, p1.y + p2.y) : make_float2(p1.x - p2.x, p1.y - p2.y);
+    };
+    // forward: even FBR columns from the sums (q = 0), odd from the
+    // differences (q = 1); k-step s2's B operand is tile columns col0 + 8 kg + e
+    auto tile_split = [&](int col0, bf8v *fre, bf8v *fim) {
+      float re[8], im[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float2 x = Xw[rl * XS + col0 + kg * 8 + e];
+        re[e] = x.x;
+        im[e] = x.y;
+      }
+      split3(re, fre);
+      split3(im, fim);
+    };
+    auto forward_products = [&](int q, int s2, const bf8v *fre, const bf8v *fim) {
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        bf8v a[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+          a[p] = *reinterpret_cast<const bf8v *>(&M[(((2 + q) * 3 + p) * NP + nt * 16 + rl) * LD + s2 * 32 + kg * 8]);
+#pragma unroll
+        for (int d = 2; d >= 0; d--)
+#pragma unroll
+          for (int p = 0; p <= d; p++) {
+            acc[q][0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[d - p], acc[q][0][nt], 0, 0, 0);
+            acc[q][1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[d - p], acc[q][1][nt], 0, 0, 0);
+          }
+      }
+    };
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < NT; b++) acc[q][a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    if constexpr (KSH == 1) {
+      // both halves into the tile, then the two products
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      if (TWOPASS && q == 1) {
-        put_tile(1);
-        wave_sync();
-      }
+      for (int q = 0; q < 2; q++)
 #pragma unroll
-      for (int s2 = 0; s2 < KSH; s2++) {
-        bf8v fre[3], fim[3];
-        {
-          float re[8], im[8];
+        for (int nt = 0; nt < NT; nt++)
 #pragma unroll
-          for (int e = 0; e < 8; e++) {
-            const float2 x = Xw[rl * XS + q * QOFF + s2 * 32 + kg * 8 + e];
-            re[e] = x.x;
-            im[e] = x.y;
-          }
-          split3(re, fre);
-          split3(im, fim);
+          for (int v = 0; v < 4; v++) Xw[rl * XS + q * KP + nt * 16 + 4 * kg + v] = sum_or_diff(q, nt, v);
+      wave_sync();
+      bf8v fre[3], fim[3];
+      tile_split(0, fre, fim);
+      forward_products(0, 0, fre, fim);
+      tile_split(KP, fre, fim);
+      wave_sync();  // reads done before the next tile's writes
+      forward_products(1, 0, fre, fim);
+    } else {
+      // one k-step's 32 columns at a time: n tiles 2 s2 and 2 s2 + 1
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int s2 = 0; s2 < KSH; s2++) {
+#pragma unroll
+          for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) Xw[rl * XS + t * 16 + 4 * kg + v] = sum_or_diff(q, 2 * s2 + t, v);
+          wave_sync();
+          bf8v fre[3], fim[3];
+          tile_split(0, fre, fim);
+          wave_sync();  // reads done before the next round's writes
+          forward_products(q, s2, fre, fim);
         }
-        // reads done before the next writes (the differences, or the next tile)
-        if (s2 + 1 == KSH && (TWOPASS || q == 1)) wave_sync();
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-          bf8v a[3];
-#pragma unroll
-          for (int p = 0; p < 3; p++)
-            a[p] = *reinterpret_cast<const bf8v *>(&M[(((2 + q) * 3 + p) * NP + nt * 16 + rl) * LD + s2 * 32 + kg * 8]);
-#pragma unroll
-          for (int d = 2; d >= 0; d--)
-#pragma unroll
-            for (int p = 0; p <= d; p++) {
-              acc[q][0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fre[d - p], acc[q][0][nt], 0, 0, 0);
-              acc[q][1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fim[d - p], acc[q][1][nt], 0, 0, 0);
-            }
-        }
-      }
     }
 #pragma unroll
     for (int q = 0; q < 2; q++)
@@ -1620,8 +1629,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_par(ThetaFusedArgs P) {
 }
 template <int KSH, int NTH>
 constexpr size_t theta_fused_par_lds() {
-  return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 +
-         (size_t)(NTH / 64) * 16 * ((KSH > 1 ? 1 : 2) * KSH * 32 + 1) * 8;
+  return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * ((KSH > 1 ? 32 : 64) + 1) * 8;
 }
 
 template <int KS, int NT>

@@ -920,17 +920,18 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
   }
 }
 
-// r pass: lines (column, i) of N = n2 points at stride n1; a batch is 256 / N1
-// consecutive i of one column (n1 a multiple of 256 / N1), so the lanes of a
-// load read consecutive addresses.  Persistent over batches, with the next
-// batch loaded while the current one is transformed.
-template <typename T, int N1, int N2>
-__global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
+// r pass: lines (column, i) of N = n2 points at stride n1; a batch is NTR / N1
+// consecutive i of one column (n1 a multiple of NTR / N1), so the lanes of a
+// load read consecutive addresses (128 B per point of the line with 256
+// threads, 256 B with 512).  Persistent over batches, with the next batch
+// loaded while the current one is transformed.
+template <typename T, int N1, int N2, int NTR>
+__global__ __launch_bounds__(NTR) void k_rline_rf(RLineArgs<T> A) {
   typedef typename cx_of<T>::type C;
   typedef typename rf_tab<T>::tw TW;
   typedef typename rf_tab<T>::kin TK;
   typedef typename rf_tab<T>::ph TB;
-  constexpr int N = N1 * N2, LPB = 256 / N1, LS = LPB + 1;
+  constexpr int N = N1 * N2, LPB = NTR / N1, LS = LPB + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TW *s_twa = reinterpret_cast<TW *>(smem_raw);
   TK *s_kin = reinterpret_cast<TK *>(s_twa + N1 * (N2 + 1));
@@ -943,7 +944,7 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   // persistent: batch b = LPB consecutive i of one column (n1 % LPB == 0);
   // the next batch's points and D_r row are loaded while this one is
   // transformed, the row through a double-buffered LDS copy
-  constexpr int PHE = (N + 255) / 256;
+  constexpr int PHE = (N + NTR - 1) / NTR;
   TB *s_ph = reinterpret_cast<TB *>(xch + N1 * N2 * LS);  // [2][N]
   const int nbatch = A.nlines / LPB;
   C raw[N2], v[N2];
@@ -960,7 +961,7 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
     for (int m = 0; m < N2; m++) raw[m] = buf_ld<C>(rd, (unsigned)(((size_t)P(m) * st + L) * sizeof(C)));
 #pragma unroll
     for (int q = 0; q < PHE; q++)
-      if (N % 256 == 0 || q * 256 + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * 256 + t) * sizeof(TB)));
+      if (N % NTR == 0 || q * NTR + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * NTR + t) * sizeof(TB)));
   };
   auto j_of = [&](int b) { return __builtin_amdgcn_readfirstlane(A.fcol_j[(b * LPB) / A.n1]); };
   int b = blockIdx.x;
@@ -974,7 +975,7 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
     TB *ph = s_ph + buf * N;
 #pragma unroll
     for (int q = 0; q < PHE; q++)
-      if (N % 256 == 0 || q * 256 + t < N) ph[q * 256 + t] = phr[q];
+      if (N % NTR == 0 || q * NTR + t < N) ph[q * NTR + t] = phr[q];
     __syncthreads();  // row (and, first time round, the tables)
     const int line = b * LPB + L;
 #pragma unroll
@@ -988,9 +989,9 @@ __global__ __launch_bounds__(256) void k_rline_rf(RLineArgs<T> A) {
   }
 }
 
-template <typename T, int N1, int N2>
+template <typename T, int N1, int N2, int NTR>
 constexpr size_t rline_rf_lds() {
-  return rf_table_bytes<T, N1, N2>() + (size_t)N1 * N2 * (256 / N1 + 1) * sizeof(typename cx_of<T>::type) +
+  return rf_table_bytes<T, N1, N2>() + (size_t)N1 * N2 * (NTR / N1 + 1) * sizeof(typename cx_of<T>::type) +
          (size_t)2 * N1 * N2 * sizeof(typename rf_tab<T>::ph);
 }
 

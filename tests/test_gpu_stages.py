@@ -106,13 +106,19 @@ def test_fused_passes_with_phases(qd, small, prec):
         ref = x * DR[:, None, :]
         assert relerr(got, ref) < TOL[prec]
         y = rnd((M, n2, n1), 22)
-        prop.set_fbr(as_f64(y))
-        prop.debug_stage(2)
-        got = as_c(prop.get_fbr(), (M, n2, n1))
         x = y * Dr[:, :, None]
         x = np.fft.ifft(np.fft.fft(x, axis=1) * np.exp(-1j * dt * T2)[None, :, None], axis=1)
         ref = x * Dr[:, :, None]
-        assert relerr(got, ref) < TOL[prec]
+        # the r pass with batches of 256 / N1 and of 512 / N1 consecutive R
+        # points: the same arithmetic per line, so the same bits
+        out = {}
+        for nth in (256, 512):
+            prop.debug_select(4, nth)
+            prop.set_fbr(as_f64(y))
+            prop.debug_stage(2)
+            out[nth] = as_c(prop.get_fbr(), (M, n2, n1)).copy()
+            assert relerr(out[nth], ref) < TOL[prec], nth
+        assert np.array_equal(out[256], out[512])
 
 
 # (R points, r points, J values, j_max): LDS-staged mixing (32 points), the
