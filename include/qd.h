@@ -121,7 +121,7 @@ typedef struct qd_options {
                              products) or 3 (six products); 0 = 3 */
   int32_t autotune;       /* 1: time the candidate kernel configurations on the
                              handle's own arrays at creation and keep the fastest */
-  int32_t fuse_theta;     /* mixed precision, angular basis of at most 64: 0 = fuse the
+  int32_t fuse_theta;     /* mixed precision, angular basis of at most 128: 0 = fuse the
                              theta-backward GEMM of each step with the dump, the flux
                              rows, the V phase and the next theta-forward GEMM (the
                              state between steps is then the FBR array; psi is rebuilt

@@ -1732,13 +1732,18 @@ static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *_
   const int xl = t & 63;
   const int eg = t >> 6;
   const int e0 = eb * FX_E;
+  // the accumulators start from the stored fai, dfai: their loads are issued
+  // first and arrive while the time block is staged and multiplied
   double2 af[2][FX_EPT], adf[2][FX_EPT];
 #pragma unroll
   for (int h = 0; h < 2; h++)
 #pragma unroll
     for (int q = 0; q < FX_EPT; q++) {
-      af[h][q] = make_double2(0.0, 0.0);
-      adf[h][q] = make_double2(0.0, 0.0);
+      const int x = xb * FX_X + xl + 64 * h, e = e0 + eg * FX_EPT + q;
+      const bool ok = x < nx && e < ne;
+      const size_t o = (size_t)e * nx + x;
+      af[h][q] = ok ? fai[o] : make_double2(0.0, 0.0);
+      adf[h][q] = ok ? dfai[o] : make_double2(0.0, 0.0);
     }
   for (int tc = 0; tc < nt; tc += FX_TC) {
     for (int u = t; u < FX_TC * FX_X; u += 256) {
@@ -1790,7 +1795,7 @@ static __global__ __launch_bounds__(256) void k_flux_accumulate(const double2 *_
         const int e = e0 + eg * FX_EPT + q;
         if (e < ne) {
           const size_t o = (size_t)e * nx + x;
-          const double2 a = cadd(fai[o], af[h][q]), b = cadd(dfai[o], adf[h][q]);
+          const double2 a = af[h][q], b = adf[h][q];
           fai[o] = a;
           dfai[o] = b;
           pim[q] = fma(w, a.x * b.y - a.y * b.x, pim[q]);  // w Im(conj(a) b)

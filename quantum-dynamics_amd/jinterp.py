@@ -295,6 +295,15 @@ def learned_partial_waves(qd, base, system, J_max: int = 40, stride: int = 4, n_
     out["held_out_max_abs_dP"] = {"network": {J: float(d_net[J].max()) for J in held},
                                   "linear_in_J": {J: float(d_lin[J].max()) for J in held}}
     if reference == "all":
+        # every skipped J judged by the propagator, not only the held-out ones
+        s_net = {J: float(np.max(np.abs(pred[J] - PJ_ref[J]))) for J in skipped}
+        s_lin = {J: float(np.max(np.abs(lin[J] - PJ_ref[J]))) for J in skipped}
+        out["skipped_max_abs_dP"] = {
+            "network": s_net, "linear_in_J": s_lin,
+            "bound": {"network": max(s_net.values()), "linear_in_J": max(s_lin.values())},
+            "mean": {"network": float(np.mean(list(s_net.values()))),
+                     "linear_in_J": float(np.mean(list(s_lin.values())))},
+            "network_better": int(sum(s_net[J] < s_lin[J] for J in skipped)), "skipped": len(skipped)}
         ref_sum = partial_wave_sum(PJ_ref)
         scale = float(np.max(np.abs(ref_sum)))
         e_net = np.abs(partial_wave_sum(pred) - ref_sum)
