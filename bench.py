@@ -39,21 +39,25 @@ HBM_PEAK_GBS = 8000.0
 # (B = s_c (2 + 6 M / n_theta) + s_v per grid-point-step: 68 B mixed, 136 B
 # FP64): theta-fwd reads psi (s_c) and V (s_v) and writes the FBR array; the R
 # and r passes and theta-bwd each read and write one complex array.
-# With the fused theta step (mixed precision, basis of at most 64) one kernel
-# per step reads the FBR array and exp(-i V dt) (8 B each) and writes the FBR
-# array: 24 B per point of a single block.  Blocks that share V (the J blocks
-# of one K) read it once per step: 16 B + 8 B x (V columns / DVR columns), 16.7 B
-# per point on the J = 0..20 batch (1071 V columns, 12551 DVR columns).
+# With the fused theta step (mixed precision, basis of at most 128) one kernel
+# per step reads the FBR array and exp(-i V dt) (8 B each), writes the FBR
+# array and the 11 r rows of the flux stencil (half-width 5) around the
+# dividing surface, 8 B x 11 / 192 r points: 24.46 B per point of a single
+# block.  Blocks that share V (the J blocks of one K) read it once per step:
+# 16 B + 8 B x (V columns / DVR columns) + the flux rows, 17.2 B per point on
+# the J = 0..20 batch (1071 V columns, 12551 DVR columns).
+FLUX_ROWS_B = 8.0 * 11 / 192
 B_ALG = {
-    "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "theta_fused": 24, "fft_R": 16, "fft_r": 16, "total": 68},
+    "mixed": {"theta_fwd": 8 + 4 + 8, "theta_bwd": 16, "theta_fused": 24 + FLUX_ROWS_B, "fft_R": 16, "fft_r": 16,
+              "total": 68},
     "fp64": {"theta_fwd": 16 + 8 + 16, "theta_bwd": 32, "fft_R": 32, "fft_r": 32, "total": 136},
 }
 # HBM bytes per launch and grid point, measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel); profiles/*.md.  "theta"
-# is the mean of theta-fwd and theta-bwd (the event class averages both).
+# WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel) over the launches of
+# timed steps (those write the flux rows); profiles/*.md.
 MEASURED_B_PER_POINT = {
-    "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 16.2},              # batched_step_kernels_FH2_J0-20.md (fused theta)
-    "single": {"theta": (24.2 + 16.2) / 2, "fft_R": 16.7, "fft_r": 16.5},    # hbm_traffic_FH2_256x192x120.md
+    "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 16.4},  # batched_step_kernels_FH2_J0-20.md (fused theta)
+    "single": {"theta": 24.7, "fft_R": 16.7, "fft_r": 16.5},   # hbm_traffic_FH2_256x192x120.md (fused theta)
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER, TK_ACC = 1, 2, 3, 4, 5, 6
 KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER),
@@ -254,10 +258,10 @@ def roofline(kern, npt, workload, fused_theta, prec="mixed"):
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_B_PER_POINT[workload][dom] * npt,
             "traffic_unit": "bytes per launch (rocprof)", "algorithmic_bytes": per_point * npt,
-            "note": ("the R pass with K<->l mixing is VALU-bound (about 154 VALU instructions per point: the "
-                     "register FFT plus the mixing in both directions; VALU issue about 65 % of its cycles), not "
-                     "HBM-bound; it moves its 16 algorithmic bytes per point" if (dom == "fft_R" and workload == "batched")
-                     else "")}
+            "note": ("the R pass with K<->l mixing is not HBM-bound: about 127 VALU instructions per point (62 the "
+                     "register FFT with its phases, 65 the mixing in both directions), VALU issuing on about 58 % of "
+                     "its cycles, and waits at the bundle barriers around the mixing; it moves its 16 algorithmic "
+                     "bytes per point (16.4 measured)" if (dom == "fft_R" and workload == "batched") else "")}
 
 
 def main():
