@@ -242,7 +242,8 @@ int qd_debug_stage(qd_handle *h, int32_t stage);
  * per work group, [6] lines per mixing bundle, [7] r-pass lines per group,
  * [8] lines per bundle of the prefetching FP32 mixing R pass (0: not used),
  * [9] its unit-size bucket, [10] fused theta step in use (1: the dense
- * transforms, 2: their parity halves), [11] its 16-row
+ * transforms, 2: their parity halves, 3: the parity halves with each tile
+ * shared by a pair of waves, one per component), [11] its 16-row
  * tiles per work group, [12] that mixing R pass runs two work groups per CU,
  * [13] threads per work group of the fused theta step, [14] the split mixing
  * R pass (K -> l mixing, the register FFT pass without mixing, l -> K mixing,
@@ -256,8 +257,9 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * when the handle does not have it, or has no other R pass for its groups.
  * what 3: the fused theta step on the dense transforms (value 0; bases of at
  * most 64) or on their parity halves (1: 512 threads per work group, 256 for
- * bases of 65..128; 2: 768 threads, bases of at most 64); QD_ERR_ARG when the
- * handle does not have that kernel.
+ * bases of 65..128; 2: 768 threads, bases of at most 64; 3: 512 threads as
+ * wave pairs, bases of 65..128); QD_ERR_ARG when the handle does not have that
+ * kernel.
  * what 4: threads per work group of the register-FFT r pass, 256 or 512
  * (batches of twice the consecutive R points); QD_ERR_ARG when it does not fit. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);

@@ -248,6 +248,8 @@ struct qd_handle {
   // the fused step on the parity halves of the theta transforms (k_theta_fused_par)
   bool tfu_par = false, par_ok = false, dense_ok = false;  // kernel in use; which kernels fit
   int par_ksh = 1;                                         // k-steps of 32 per half basis
+  // half bases of 33..64: each tile shared by a pair of waves (k_theta_fused_pair)
+  bool tfu_pair = false, pair_ok = false;
   void *Wpar = nullptr;
   long long par_stride = 0;
   // fused theta kernel: groups of blocks (in d_blocks_k order) sharing the theta
@@ -609,8 +611,9 @@ int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
   F.pre_dump = h->variant == QD_VARIANT_CUDA ? 1 : 0;
   F.items = h->d_tfu_items;
   size_t lds = 0;
-  const void *fn = h->tfu_par ? theta_fused_par_kernel(h->par_ksh, h->tfu_nth, &lds)
-                              : theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
+  const void *fn = h->tfu_pair ? theta_fused_pair_kernel(h->tfu_nth, &lds)
+                   : h->tfu_par ? theta_fused_par_kernel(h->par_ksh, h->tfu_nth, &lds)
+                                : theta_fused_kernel(h->tmx_ks, h->tfu_nth, &lds);
   if (!fn) return fail(QD_ERR_ARG, "no fused theta kernel for this basis size");
   if (h->tfu_nitems < 1) return fail(QD_ERR_ARG, "fused theta work items not set up");
   dim3 grid((unsigned)h->tfu_nitems, 1, 1);
@@ -625,7 +628,9 @@ int launch_theta_fused(qd_handle *h, bool rows = true, bool all = false) {
 // about tfu_tpw tile-blocks.  Set-up and autotuning only (a synchronous copy).
 int build_tfu_items(qd_handle *h) {
   const long long ntiles = ((long long)h->n12 + 15) / 16;
-  const std::vector<TfuItem> items = qd::build_tfu_items(h->tfu_groups, ntiles, h->tfu_tpw, h->tfu_nth / 64);
+  // tiles taken side by side: one per wave, or one per wave pair
+  const int lanes = h->tfu_pair ? h->tfu_nth / 128 : h->tfu_nth / 64;
+  const std::vector<TfuItem> items = qd::build_tfu_items(h->tfu_groups, ntiles, h->tfu_tpw, lanes);
   const std::string err = check_tfu_items(items, h->tfu_blocks, (long long)h->n12, h->ndcol, h->nfcol, h->nV);
   if (!err.empty()) return fail(QD_ERR_ARG, err);
   if ((int)items.size() > h->tfu_items_cap) {
@@ -1726,11 +1731,16 @@ int create_batch(qd_handle *h, const BatchSpec &S) {
         if (ksh == 1) (void)fits(theta_fused_par_kernel(1, 768, &plds), 768, plds);
         h->par_ok = true;
         h->par_ksh = ksh;
+        h->pair_ok = ksh == 2 && fits(theta_fused_pair_kernel(512, &plds), 512, plds);
       }
       if (h->dense_ok || h->par_ok) {
+        // default: the parity halves, as wave pairs where they fit (half
+        // bases of 33..64: 93.4 against 96.6 us per step on a 256 x 192 x 120
+        // grid)
         h->fuse_theta = true;
         h->tfu_par = h->par_ok;
-        h->tfu_nth = h->par_ok ? pnth : TMX_THREADS;
+        h->tfu_pair = h->pair_ok;
+        h->tfu_nth = h->pair_ok ? 512 : h->par_ok ? pnth : TMX_THREADS;
         h->tfu_tpw = h->tmx_tpw;
         QD_RC(build_tfu_items(h));
       }
@@ -2134,26 +2144,42 @@ int autotune_handle(qd_handle *h) {
   // fused theta kernel: tile-blocks per work item (items of blocks sharing V
   // are at least one tile per wave, so smaller values matter too)
   if (h->fuse_theta) {
-    // kernel: the dense transforms (512 threads) or the parity halves (512 or
-    // 768 threads), at the starting tile count
-    if (h->par_ok && (h->dense_ok || h->par_ksh == 1)) {
-      const int cpar[3] = {0, 1, 1}, cnth[3] = {TMX_THREADS, 512, 768};
-      float best = 1e30f;
-      int pick = 1;
-      for (int c = 0; c < 3; c++) {
-        if ((cpar[c] == 0 && !h->dense_ok) || (cpar[c] == 1 && h->par_ksh != 1)) continue;
-        h->tfu_par = cpar[c] == 1;
-        h->tfu_nth = cnth[c];
-        QD_RC(build_tfu_items(h));
-        float ms = 0;
-        QD_RC(time_launches(h, [&] { return launch_theta_fused(h, false); }, reps, &ms));
-        if (ms < best) {
-          best = ms;
-          pick = c;
-        }
+    // kernel, at the starting tile count: the dense transforms (512 threads),
+    // the parity halves (512 or 768 threads for half bases of at most 32, 256
+    // above), or the parity halves with wave pairs (512 threads, half bases of
+    // 33..64)
+    if (h->par_ok) {
+      struct Cand {
+        int kind, nth;  // kind 0 dense, 1 parity halves, 2 wave pairs
+      };
+      std::vector<Cand> cands;
+      if (h->dense_ok) cands.push_back({0, TMX_THREADS});
+      if (h->par_ksh == 1) {
+        cands.push_back({1, 512});
+        cands.push_back({1, 768});
+      } else {
+        cands.push_back({1, 256});
       }
-      h->tfu_par = cpar[pick] == 1;
-      h->tfu_nth = cnth[pick];
+      if (h->pair_ok) cands.push_back({2, 512});
+      if (cands.size() > 1) {
+        float best = 1e30f;
+        size_t pick = 0;
+        for (size_t c = 0; c < cands.size(); c++) {
+          h->tfu_par = cands[c].kind >= 1;
+          h->tfu_pair = cands[c].kind == 2;
+          h->tfu_nth = cands[c].nth;
+          QD_RC(build_tfu_items(h));
+          float ms = 0;
+          QD_RC(time_launches(h, [&] { return launch_theta_fused(h, false); }, reps, &ms));
+          if (ms < best) {
+            best = ms;
+            pick = c;
+          }
+        }
+        h->tfu_par = cands[pick].kind >= 1;
+        h->tfu_pair = cands[pick].kind == 2;
+        h->tfu_nth = cands[pick].nth;
+      }
     }
     const long long ntiles = (h->n12 + 15) / 16;
     const int base = h->tfu_tpw;
@@ -2299,7 +2325,7 @@ int qd_get_tuning(qd_handle *h, int32_t *vals) {
   vals[7] = h->rl_rf ? h->rl_nth / rf_n1(h->n2) : h->rl_lpb;
   vals[8] = h->rp_mx ? std::max(h->mxs[0].nl, h->mxs[1].nl) : 0;
   vals[9] = h->rp_mx ? std::max(h->mxs[0].gb, h->mxs[1].gb) : 0;
-  vals[10] = h->fuse_theta ? (h->tfu_par ? 2 : 1) : 0;
+  vals[10] = h->fuse_theta ? (h->tfu_pair ? 3 : h->tfu_par ? 2 : 1) : 0;
   vals[11] = h->fuse_theta ? h->tfu_tpw : 0;
   vals[13] = h->fuse_theta ? h->tfu_nth : 0;
   vals[12] = h->rp_mx && h->mx_two ? 1 : 0;
@@ -2672,12 +2698,14 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
   }
   if (what == 3) {
     if (!h->fuse_theta) return fail(QD_ERR_ARG, "this handle has no fused theta step");
-    if (value < 0 || value > 2) return fail(QD_ERR_ARG, "value must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(QD_ERR_ARG, "value must be 0, 1, 2 or 3");
     if (value > 0 && !h->par_ok) return fail(QD_ERR_ARG, "this handle's theta matrices have no parity halves");
     if (value == 0 && !h->dense_ok) return fail(QD_ERR_ARG, "the dense fused kernel does not fit this basis");
     if (value == 2 && h->par_ksh != 1) return fail(QD_ERR_ARG, "768 threads only for half bases of at most 32");
+    if (value == 3 && !h->pair_ok) return fail(QD_ERR_ARG, "wave pairs only for half bases of 33..64");
     drop_graph(h);
     h->tfu_par = value > 0;
+    h->tfu_pair = value == 3;
     h->tfu_nth = value == 2 ? 768 : value == 1 && h->par_ksh == 2 ? 256 : TMX_THREADS;
     return build_tfu_items(h);
   }

@@ -245,6 +245,14 @@ const void *theta_fused_par_kernel(int ksh, int nth, size_t *lds) {
 #undef QD_TFP
   return nullptr;
 }
+
+// fused kernel on the parity halves with each tile shared by a pair of waves
+// (one per component): half bases of 33..64
+const void *theta_fused_pair_kernel(int nth, size_t *lds) {
+  if (nth != 512) return nullptr;
+  *lds = theta_fused_pair_lds<512>();
+  return reinterpret_cast<const void *>(&k_theta_fused_pair<512>);
+}
 #endif  // theta part
 
 #if QD_PART == 5 || QD_PART == 6

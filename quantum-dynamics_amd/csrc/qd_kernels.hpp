@@ -1632,6 +1632,238 @@ constexpr size_t theta_fused_par_lds() {
   return (size_t)4 * 3 * (2 * KSH * 16) * (KSH * 32 + 8) * 2 + (size_t)(NTH / 64) * 16 * ((KSH > 1 ? 32 : 64) + 1) * 8;
 }
 
+__device__ __forceinline__ float buf_ld_f(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buf_st_f(float x, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)off, 0, 0);
+}
+
+// The fused theta step on the parity halves for half bases of 33..64 (two
+// k-steps per half) with each 16-point tile shared by a pair of waves: wave
+// 2 p + h computes component h (0: real, 1: imaginary) of every product of
+// pair p's tiles.  The transforms are real, so the two components' products
+// are independent; only the V phase couples them, and it is applied where the
+// pair meets anyway: each wave writes its component of the DVR pair values
+// (after the dump) into the pair's LDS tile, and both read complex values back
+// in the layout of the forward B operand (DVR rows 8 kg + e of a k-step at
+// point rl), one k-step (32 DVR rows, with their mirrors) per round.  Per wave
+// this halves the accumulators and B operands of k_theta_fused_par<2, *>:
+// eight waves (two per SIMD) fit in 256 registers, where one wave per tile
+// needs about 400 and leaves one wave per SIMD.  Loads and stores move one
+// float component (4 bytes at an 8-byte stride; the pair's two halves meet
+// in the caches).  The pair meets at work-group barriers, so every wave runs
+// the same number of (tile, block) iterations; tiles past the item's end are
+// masked (loads read zeros, stores are dropped).  Formulas, middle node,
+// dump, flux rows and both propagator orders as in k_theta_fused_par.
+template <int NTH>
+__global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
+  extern __shared__ __align__(16) unsigned char tm_lds[];
+  constexpr int KSH = 2, KP = 64, NT = 4, NP = 64, LD = KP + 8, XS = 2 * 32 + 1, NPR = NTH / 128;
+  __bf16 *M = reinterpret_cast<__bf16 *>(tm_lds);  // [4][3][NP][LD]
+  // wave-uniform values in scalar registers (buffer descriptors built from
+  // them must be, or every access becomes a loop over lanes)
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), pr = wave >> 1,
+            h = wave & 1;
+  // the pair's tile: [16 points][XS]; columns 0..31 DVR rows 32 s2 + j, 32..63 their mirrors
+  float2 *Xp = reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + pr * 16 * XS;
+  float *Xf = reinterpret_cast<float *>(Xp);
+  const int4 item = P.items[blockIdx.x];
+  const int gb0 = item.x, gnb = item.y, t0 = item.z, t1 = item.w;
+  const BlockDesc bd = P.blocks[gb0];  // the group shares mat, nd, nf and the V columns
+  const int R = P.R, nd = bd.nd, nf = bd.nf;
+  const int NH = (nd + 1) >> 1, CE = (nf + 1) >> 1, CO = nf >> 1;
+  auto rL_of = [&](int b) {
+    const int f0 = __builtin_amdgcn_readfirstlane(P.blocks[gb0 + b].fcol0);
+    return buf_rsrc(P.L + (size_t)f0 * R, (unsigned)nf * R * 8u);
+  };
+  auto rP_of = [&](int b) {
+    const int d0 = __builtin_amdgcn_readfirstlane(P.blocks[gb0 + b].dcol0);
+    return buf_rsrc(P.psi + (size_t)d0 * R, (unsigned)nd * R * 8u);
+  };
+  const __amdgpu_buffer_rsrc_t rV = buf_rsrc(P.expV + P.dcol_vidx[bd.dcol0], (unsigned)nd * R * 8u);
+  const int rl = lane & 15, kg = lane >> 4;
+  const unsigned hoff = 4u * (unsigned)h;
+  float ra[2][8];  // [parity][e]: component h of FBR column 2 (32 s + 8 kg + e) + parity
+  auto fetch = [&](const __amdgpu_buffer_rsrc_t &rd, int row, bool ok, int s, int Rx) {
+    const int k0 = s * 32 + kg * 8;
+    const unsigned base = ok ? (unsigned)row * 8u + hoff : TM_OOB;
+    asm volatile("" : "+s"(Rx));
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int c = 2 * (k0 + e) + q;
+        ra[q][e] = buf_ld_f(rd, c < nf ? base + (unsigned)(Rx * c) * 8u : TM_OOB);
+      }
+  };
+  auto split3 = [](const float *x, bf8v *out) {
+    float r[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = x[e];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const __bf16 b = f2bf(r[e]);
+        out[q][e] = b;
+        r[e] -= (float)b;
+      }
+  };
+  // (tile, block) iterations: the same count for every wave of the group
+  const int nit = (t1 - t0 + NPR - 1) / NPR * gnb;
+  int tile = t0 + pr, bi = 0;
+  __amdgpu_buffer_rsrc_t rL = rL_of(0);
+  // the first tile's data are on their way while the matrices are copied
+  fetch(rL, tile * 16 + rl, tile < t1 && tile * 16 + rl < R, 0, R);
+  {
+    const __bf16 *G = reinterpret_cast<const __bf16 *>(P.Bb) + (size_t)bd.mat * P.mat_stride;
+    constexpr int QV = KP / 8;
+    for (int u = threadIdx.x; u < 4 * 3 * NP * QV; u += blockDim.x) {
+      const int pn = u / QV, kq = u % QV;
+      *reinterpret_cast<bf8v *>(&M[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&G[(size_t)pn * KP + kq * 8]);
+    }
+  }
+  __syncthreads();
+  for (int it = 0; it < nit; it++) {
+    // R as an opaque per-iteration value: the offsets R c 8 are formed where
+    // they are used instead of being hoisted out of the loop (about a hundred
+    // of them would not fit in scalar registers)
+    int Ri = R;
+    asm volatile("" : "+s"(Ri));
+    const int row = tile * 16 + rl;
+    const bool rin = tile < t1 && row < R;
+    const bool lastb = bi + 1 == gnb;
+    const int ntile = lastb ? tile + NPR : tile, nbi = lastb ? 0 : bi + 1;
+    const int nrow = ntile * 16 + rl;
+    const bool nin = ntile < t1 && nrow < R;
+    const __amdgpu_buffer_rsrc_t rLn = rL_of(nbi);
+    f4v acc[2][NT];  // [parity][n tile]: component h of E and O
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int b = 0; b < NT; b++) acc[q][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    // backward: E and O over the even and odd FBR columns
+#pragma unroll 1
+    for (int s = 0; s < KSH; s++) {
+      bf8v bx[2][3];
+#pragma unroll
+      for (int q = 0; q < 2; q++) split3(ra[q], bx[q]);
+      // the next k-step, or the next (tile, block)'s first: the block is
+      // selected as an index, so that the descriptor stays scalar
+      const bool more = s + 1 < KSH;
+      fetch(rL_of(more ? bi : nbi), more ? row : nrow, more ? rin : nin, more ? s + 1 : 0, Ri);
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+          bf8v a[3];
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+            a[p] = *reinterpret_cast<const bf8v *>(&M[((q * 3 + p) * NP + nt * 16 + rl) * LD + s * 32 + kg * 8]);
+#pragma unroll
+          for (int d = 2; d >= 0; d--)
+#pragma unroll
+            for (int p = 0; p <= d; p++)
+              acc[q][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bx[q][d - p], acc[q][nt], 0, 0, 0);
+        }
+    }
+    float dfac = 1.f;
+    if (P.d1f && rin) dfac = P.d1f[row % P.n1] * P.d2f[row / P.n1];
+    const int rr = row / P.n1;
+    const bool frow = rin && rr >= P.r_lo && rr <= P.r_hi;
+    const __amdgpu_buffer_rsrc_t rP = rP_of(bi);
+    f4v accf[2][NT];  // [parity][n tile]: component h of the forward products
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int b = 0; b < NT; b++) accf[q][b] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s2 = 0; s2 < KSH; s2++) {
+      // V phase at the forward B-operand rows i = 32 s2 + 8 kg + e and their
+      // mirrors, read for each block and round (registers are short here)
+      float2 wv[8][2];
+      int Rv = Ri;
+      asm volatile("" : "+s"(Rv));
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int i = 32 * s2 + 8 * kg + e;
+        const bool ok = rin && i < NH;
+        wv[e][0] = buf_ld_f2(rV, ok ? (unsigned)(row + Rv * i) * 8u : TM_OOB);
+        wv[e][1] = buf_ld_f2(rV, ok ? (unsigned)(row + Rv * (nd - 1 - i)) * 8u : TM_OOB);
+      }
+      // DVR rows i = 32 s2 .. 32 s2 + 31 (n tiles 2 s2, 2 s2 + 1): the pair
+      // (i, i') after the dump, psi rows, then component h into the tile
+      int Rf = Ri;
+      asm volatile("" : "+s"(Rf));
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int nt = 2 * s2 + t, j = t * 16 + 4 * kg + v, i = 32 * s2 + j, im = nd - 1 - i;
+          const float e1 = acc[0][nt][v] + acc[1][nt][v], e2 = acc[0][nt][v] - acc[1][nt][v];
+          const float c1 = e1 * dfac, c2 = e2 * dfac;
+          if (frow) {
+            buf_st_f(P.pre_dump ? e1 : c1, rP, i < NH ? (unsigned)(row + Rf * i) * 8u + hoff : TM_OOB);
+            buf_st_f(P.pre_dump ? e2 : c2, rP, i < nd / 2 ? (unsigned)(row + Rf * im) * 8u + hoff : TM_OOB);
+          }
+          Xf[(rl * XS + j) * 2 + h] = c1;
+          Xf[(rl * XS + 32 + j) * 2 + h] = c2;
+        }
+      __syncthreads();
+      // V phase at the B-operand positions, component h; sums and differences
+      float sm[8], df[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float2 x1 = Xp[rl * XS + 8 * kg + e], x2 = Xp[rl * XS + 32 + 8 * kg + e];
+        const float2 w1 = wv[e][0], w2 = wv[e][1];
+        const float p1 = h ? x1.x * w1.y + x1.y * w1.x : x1.x * w1.x - x1.y * w1.y;
+        const float p2 = h ? x2.x * w2.y + x2.y * w2.x : x2.x * w2.x - x2.y * w2.y;
+        sm[e] = p1 + p2;
+        df[e] = p1 - p2;
+      }
+      __syncthreads();  // reads done before the next round's writes
+      // forward k-step s2: even FBR columns from the sums, odd from the differences
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        bf8v fb[3];
+        split3(q ? df : sm, fb);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+          bf8v a[3];
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+            a[p] = *reinterpret_cast<const bf8v *>(&M[(((2 + q) * 3 + p) * NP + nt * 16 + rl) * LD + s2 * 32 + kg * 8]);
+#pragma unroll
+          for (int d = 2; d >= 0; d--)
+#pragma unroll
+            for (int p = 0; p <= d; p++)
+              accf[q][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], fb[d - p], accf[q][nt], 0, 0, 0);
+        }
+      }
+    }
+    int Ro = Ri;
+    asm volatile("" : "+s"(Ro));
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int cq = nt * 16 + 4 * kg + v, c = 2 * cq + q;
+          const bool ok = rin && cq < (q ? CO : CE);
+          buf_st_f(accf[q][nt][v], rL, ok ? (unsigned)(row + Ro * c) * 8u + hoff : TM_OOB);
+        }
+    tile = ntile;
+    bi = nbi;
+    rL = rLn;
+  }
+}
+template <int NTH>
+constexpr size_t theta_fused_pair_lds() {
+  return (size_t)4 * 3 * 64 * 72 * 2 + (size_t)(NTH / 128) * 16 * 65 * 8;
+}
+
 template <int KS, int NT>
 constexpr size_t theta_mx_lds() {
   return (size_t)3 * NT * 16 * (KS * 32 + 8) * 2;

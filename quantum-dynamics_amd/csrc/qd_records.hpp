@@ -218,7 +218,7 @@ struct TfuBlock {
 // Every index k_theta_fused forms from its items: the blocks of an item share
 // matrices, sizes and V; its tiles lie in the grid; each block's columns lie in
 // the DVR / FBR arrays (ndcol / nfcol columns of n12 points) and V (nV
-// values); the per-block byte counts of its buffer descriptors fit 32 bits.
+// values); the per-block byte counts of its buffer descriptors stay below 2^31.
 // Also checks that each block gets as many tiles as it has.
 inline std::string check_tfu_items(const std::vector<TfuItem> &items, const std::vector<TfuBlock> &blocks,
                                    long long n12, long long ndcol, long long nfcol, long long nV) {
@@ -242,8 +242,10 @@ inline std::string check_tfu_items(const std::vector<TfuItem> &items, const std:
       if (b.nd < 1 || b.nf < 1 || b.dcol0 < 0 || b.fcol0 < 0 || b.dcol0 + b.nd > ndcol || b.fcol0 + b.nf > nfcol)
         return bad("block columns outside the DVR or FBR array", i);
       if (b.vidx < 0 || b.vidx + (long long)b.nd * n12 > nV) return bad("V columns outside V", i);
-      if ((unsigned long long)b.nd * n12 * 8 >= (1ULL << 32) || (unsigned long long)b.nf * n12 * 8 >= (1ULL << 32))
-        return bad("a block's byte count exceeds a buffer descriptor's 32-bit range", i);
+      // below 2^31: masked lanes use offsets from 2^31 up, which must read as
+      // out of range
+      if ((unsigned long long)b.nd * n12 * 8 >= (1ULL << 31) || (unsigned long long)b.nf * n12 * 8 >= (1ULL << 31))
+        return bad("a block's byte count reaches 2^31, the kernels' out-of-range offset", i);
       covered[it.gb0 + k] += it.t1 - it.t0;
     }
   }

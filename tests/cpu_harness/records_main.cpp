@@ -150,8 +150,10 @@ int main(int argc, char **argv) {
   }
   const long long ntiles = (n12 + 15) / 16;
   size_t nitems = 0;
+  // tiles side by side per work group: 4 (256 threads, or 512 as wave pairs), 8, 12 (768 threads)
+  for (int lanes : {4, 8, 12})
   for (int tpw : {1, 9, 64, 150, 1000, 100000}) {
-    std::vector<qd::TfuItem> items = qd::build_tfu_items(groups, ntiles, tpw, 8);
+    std::vector<qd::TfuItem> items = qd::build_tfu_items(groups, ntiles, tpw, lanes);
     if (plant == 4) items.back().t1 += 1;                                // a tile past the grid
     if (plant == 5 && items.size() > 1) items.pop_back();                // tiles left uncovered
     const std::string err = qd::check_tfu_items(items, tb, n12, nfcol, nfcol, nV);

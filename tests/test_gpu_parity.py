@@ -155,11 +155,12 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
     level of the mixed path, and are no further from the FP64 path than the
     unfused ones.  Each fused kernel is checked: the dense transforms, and
     their parity halves with 512 and 768 threads per work group; on a
-    101-point theta grid (half bases of 51 and an odd middle node) the only
-    fused kernel, the parity halves with two k-steps per half."""
+    101-point theta grid (half bases of 51 and an odd middle node) the parity
+    halves with two k-steps per half, one wave per tile or a pair of waves
+    (one per component) per tile."""
     pb = qd.problem
     print("case: fused theta vs unfused,", kind, variant, flush=True)
-    kernels = (1,) if kind == "wide" else (0, 1, 2)
+    kernels = (1, 3) if kind == "wide" else (0, 1, 2)
     if kind in ("single", "wide"):
         prob = pb.config(0, steps=60) if kind == "single" else pb.config(0, ntheta=101, steps=60)
         mk = lambda fuse: qd.Propagator(prob.copy(), precision="mixed", variant=variant, fuse_theta=fuse)
@@ -177,9 +178,10 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
             if fuse is False:
                 assert prop.tuning()["fused_theta"] == 0
             elif fuse != "fp64":
-                assert prop.tuning()["fused_theta"] == 2  # parity halves by default
+                # parity halves by default, as wave pairs for half bases of 33..64
+                assert prop.tuning()["fused_theta"] == (3 if kind == "wide" else 2)
                 prop.debug_select(3, fuse)
-                assert prop.tuning()["fused_theta"] == (1 if fuse == 0 else 2)
+                assert prop.tuning()["fused_theta"] == {0: 1, 1: 2, 2: 2, 3: 3}[fuse]
             prop.propagate(25)
             n_mid = prop.norm()
             prop.propagate(35)
