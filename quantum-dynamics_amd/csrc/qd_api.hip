@@ -16,6 +16,10 @@
 //   k_flux_extract            psi_s, dpsi_s -> time-block slot
 // and every flux_block steps k_flux_accumulate folds the block into the
 // energy-resolved accumulators (CRPAux.F:4-31).  P(E) is evaluated on demand.
+// With the fused theta step (R pass, r pass, theta step) the extraction of
+// step n runs in the first work groups of step n + 1's register-FFT r pass,
+// beside its line transforms; a fold, a read of psi and the end of a
+// propagate call run it on its own first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -181,6 +185,10 @@ struct qd_handle {
   long long acc_steps = 0;            // host mirror of d_acc_steps
   long long *d_acc_steps = nullptr;   // steps already accumulated into fai (device)
   int slot = 0;
+  // the last fused step's flux-surface extraction, not yet run (it goes into
+  // the next r pass, or runs on its own before anything reads its slot or psi)
+  bool ext_pending = false;
+  int ext_slot = 0;
   // captured flux block (tblock steps + accumulation) for qd_options.use_graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -479,8 +487,44 @@ int launch_rpass(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   return QD_OK;
 }
 
+// arguments of the flux-surface extraction into time-block slot `slot`
 template <typename T>
-int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
+FluxExtArgs<T> flux_ext_args(const qd_handle *h, int slot) {
+  FluxExtArgs<T> E;
+  E.psi = (const typename cx_of<T>::type *)(sizeof(T) == 8 ? (const void *)h->psi64 : (const void *)h->psi32);
+  E.bpsi = h->bpsi;
+  E.bdpsi = h->bdpsi;
+  E.n1 = h->n1;
+  E.n2 = h->n2;
+  E.ncols = (int)h->ndcol;
+  E.s = h->surf;
+  E.slot = slot;
+  E.dr = h->dr2;
+  E.st = h->stencil;
+  E.nwg = 0;
+  return E;
+}
+
+template <typename T>
+int launch_flux_extract(qd_handle *h, int slot) {
+  hipLaunchKernelGGL(k_flux_extract<T>, grid1(h->nx), dim3(256), 0, h->stream, flux_ext_args<T>(h, slot));
+  QD_HIP(hipGetLastError());
+  return QD_OK;
+}
+
+// runs the deferred extraction of the last fused step on its own
+int resolve_ext(qd_handle *h) {
+  if (!h->ext_pending) return QD_OK;
+  h->ext_pending = false;
+  QD_RC(mark(h, TK_FLUX));
+  return h->prec == QD_PREC_FP64 ? launch_flux_extract<double>(h, h->ext_slot)
+                                 : launch_flux_extract<float>(h, h->ext_slot);
+}
+
+// ext_slot >= 0: the register-FFT pass also extracts the flux surface into
+// that time-block slot, in work groups after its own
+template <typename T>
+int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data, int ext_slot = -1) {
   typedef typename cx_of<T>::type C;
   RLineArgs<T> a;
   a.data = data;
@@ -499,17 +543,20 @@ int launch_rline(qd_handle *h, int flags, typename cx_of<T>::type *data) {
   a.nlines = h->n1 * h->nfcol;
   a.flags = flags;
   a.plan = h->planr;
+  a.ext = flux_ext_args<T>(h, ext_slot);
   if (h->rl_rf && (flags & QD_RF_FLAGS) == QD_RF_FLAGS) {
     size_t lds = 0;
     int lpb = 0;
     const void *fn = rline_rf_kernel<T>(h->n2, h->rl_nth, &lds, &lpb);
+    if (ext_slot >= 0) a.ext.nwg = (int)((h->nx + h->rl_nth - 1) / h->rl_nth);
     void *args[] = {&a};
     const long long slots = h->rl_nth == 512 ? h->rl_slots2 : h->rl_slots;
     const unsigned nblk = (unsigned)std::min(slots, ((long long)h->n1 * h->nfcol + lpb - 1) / lpb);
-    QD_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(h->rl_nth), args, lds, h->stream));
+    QD_HIP(hipLaunchKernel(fn, dim3(nblk + (unsigned)a.ext.nwg), dim3(h->rl_nth), args, lds, h->stream));
     QD_HIP(hipGetLastError());
     return QD_OK;
   }
+  if (ext_slot >= 0) QD_RC(launch_flux_extract<T>(h, ext_slot));
   const void *fn = h->rl_ct ? rline_kernel<T>(h->n2, h->rl_lpb) : nullptr;
   if (fn) {
     void *args[] = {&a};
@@ -663,6 +710,7 @@ int enter_fbr_state(qd_handle *h) {
 // callback point writes all of psi instead (step_impl), and psi is always
 // valid where it is read.
 int materialize_psi(qd_handle *h) {
+  QD_RC(resolve_ext(h));  // it reads the psi rows this would overwrite
   if (!h->fuse_theta || !h->state_fbr || h->dvr_valid) return QD_OK;
   if (h->variant == QD_VARIANT_CUDA) return fail(QD_ERR_ARG, "internal: psi of the CUDA-order fused state is not held");
   QD_RC(launch_theta(h, false, 0, 2));
@@ -671,6 +719,7 @@ int materialize_psi(qd_handle *h) {
 }
 
 int flush_flux(qd_handle *h) {
+  QD_RC(resolve_ext(h));
   if (!h->do_crp || h->slot == 0) return QD_OK;
   QD_RC(mark(h, TK_ACC));
   hipLaunchKernelGGL(k_flux_tau, grid1((size_t)h->slot * h->ne), dim3(256), 0, h->stream, h->ftau, h->energies, h->ne,
@@ -829,6 +878,7 @@ int step_impl(qd_handle *h, bool diag = false, bool psi_out = false) {
   if (diag) {
     // the diagnostics need the intermediate arrays: this step runs one kernel
     // per operator on psi; a fused-theta handle returns to its FBR state after it
+    QD_RC(resolve_ext(h));
     QD_RC(materialize_psi(h));
     h->state_fbr = false;
     QD_RC(plain_step<T>(h, true));
@@ -841,7 +891,10 @@ int step_impl(qd_handle *h, bool diag = false, bool psi_out = false) {
     QD_RC(mark(h, TK_FFTR1));
     QD_RC(launch_rpass<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
     QD_RC(mark(h, TK_FFTR2));
-    QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi));
+    // the previous step's extraction (if pending) rides along with the r pass
+    const int ext = h->ext_pending ? h->ext_slot : -1;
+    h->ext_pending = false;
+    QD_RC(launch_rline<T>(h, FFT_PHASE | FFT_FWD | FFT_KIN | FFT_INV, lpsi, ext));
     QD_RC(mark(h, TK_THETA));
     QD_RC(launch_theta_fused(h, true, all));
     h->dvr_valid = all;
@@ -861,10 +914,15 @@ int step_impl(qd_handle *h, bool diag = false, bool psi_out = false) {
   }
   QD_HIP(hipGetLastError());
   if (h->do_crp) {
-    QD_RC(mark(h, TK_FLUX));
-    hipLaunchKernelGGL(k_flux_extract<T>, grid1(h->nx), dim3(256), 0, h->stream, (const C *)psi, h->bpsi, h->bdpsi,
-                       h->n1, h->n2, h->ndcol, h->surf, h->dr2, h->stencil, h->slot);
-    QD_HIP(hipGetLastError());
+    if (h->fuse_theta && !diag && h->rl_rf) {
+      // deferred to the next r pass (or run on its own by flush_flux,
+      // materialize_psi or the end of the call)
+      h->ext_pending = true;
+      h->ext_slot = h->slot;
+    } else {
+      QD_RC(mark(h, TK_FLUX));
+      QD_RC(launch_flux_extract<T>(h, h->slot));
+    }
     h->slot++;
     if (h->slot == h->tblock) QD_RC(flush_flux(h));
   }
@@ -1973,6 +2031,7 @@ void release(qd_handle *h) {
 int capture_block(qd_handle *h) {
   const int slot0 = h->slot;
   const long long acc0 = h->acc_steps;
+  const bool ext0 = h->ext_pending;  // false: a block starts at slot 0, after a fold
   QD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = QD_OK;
   // the block's last step leaves psi valid (a CUDA-order fused handle)
@@ -1983,6 +2042,7 @@ int capture_block(qd_handle *h) {
   const hipError_t e = hipStreamEndCapture(h->stream, &g);
   h->slot = slot0;
   h->acc_steps = acc0;
+  h->ext_pending = ext0;
   if (rc != QD_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -2060,6 +2120,7 @@ int propagate_impl(qd_handle *h, int32_t nsteps) {
       if (h->cb && h->cb(h->steps, c, h->cb_user)) break;
     }
   }
+  QD_RC(resolve_ext(h));  // the last step's extraction: psi may be read next
   if (h->profile) {
     QD_RC(mark(h, -1));
     QD_RC(drain_marks(h));

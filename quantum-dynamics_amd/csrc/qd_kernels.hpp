@@ -574,6 +574,55 @@ __global__ __launch_bounds__(RP_THREADS) void k_rpass(RPassArgs<T> A) {
 // the kinetic factor and (when the group lies in one column) its D_r/2 row
 // are staged in LDS.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Flux surface: psi_s and d psi_s / dr on r = r_s (Gradient3.F:9-167,
+// gradients2.cu:583-610) for every DVR column, into slot `slot` of the time block.
+// ---------------------------------------------------------------------------
+struct StencilArg {
+  int h;
+  double num[31];  // numerators c_j (integer form for 3..11 points) ...
+  double den;      // ... over den: d psi = sum_j num_j (f(s+j) - f(s-j)) / (den dr)
+};
+
+// the extraction's arguments; nwg > 0: the first nwg work groups of a
+// register-FFT r pass run it instead of transforming lines
+template <typename T>
+struct FluxExtArgs {
+  const typename cx_of<T>::type *psi;
+  double2 *bpsi, *bdpsi;
+  int n1, n2, ncols, s, slot;
+  double dr;
+  StencilArg st;
+  int nwg;
+};
+
+// surface point x (R index i, DVR column k) of the extraction
+template <typename T>
+__device__ __forceinline__ void flux_extract_at(const FluxExtArgs<T> &E, int x) {
+  const int nx = E.n1 * E.ncols;
+  if (x >= nx) return;
+  const int i = x % E.n1, k = x / E.n1;
+  const size_t base = (size_t)i + (size_t)E.n1 * E.n2 * k;
+  auto F = [&](int j) {
+    const typename cx_of<T>::type v = E.psi[base + (size_t)E.n1 * j];
+    return make_double2((double)v.x, (double)v.y);
+  };
+  double2 acc = make_double2(0.0, 0.0);
+  for (int j = E.st.h; j >= 1; j--) {
+    const double2 a = F(E.s + j), b = F(E.s - j);
+    acc.x += E.st.num[j - 1] * (a.x - b.x);
+    acc.y += E.st.num[j - 1] * (a.y - b.y);
+  }
+  const double den = E.st.den * E.dr;
+  E.bpsi[(size_t)E.slot * nx + x] = F(E.s);
+  E.bdpsi[(size_t)E.slot * nx + x] = make_double2(acc.x / den, acc.y / den);
+}
+
+template <typename T>
+__global__ void k_flux_extract(FluxExtArgs<T> E) {
+  flux_extract_at<T>(E, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 template <typename T>
 struct RLineArgs {
   typename cx_of<T>::type *data;
@@ -590,6 +639,9 @@ struct RLineArgs {
   const typename rf_tab<T>::tw *ltw;
   const typename rf_tab<T>::kin *lkin;
   const typename rf_tab<T>::ph *lph;
+  // the previous step's flux-surface extraction, run by the first ext.nwg
+  // work groups of the register-FFT pass (nwg 0: none)
+  FluxExtArgs<T> ext;
 };
 
 template <typename T>
@@ -1869,39 +1921,8 @@ constexpr size_t theta_mx_lds() {
   return (size_t)3 * NT * 16 * (KS * 32 + 8) * 2;
 }
 
-// ---------------------------------------------------------------------------
-// Flux surface: psi_s and d psi_s / dr on r = r_s (Gradient3.F:9-167,
-// gradients2.cu:583-610) for every DVR column, into slot `slot` of the time block.
-// ---------------------------------------------------------------------------
-struct StencilArg {
-  int h;
-  double num[31];  // numerators c_j (integer form for 3..11 points) ...
-  double den;      // ... over den: d psi = sum_j num_j (f(s+j) - f(s-j)) / (den dr)
-};
-
-template <typename T>
-__global__ void k_flux_extract(const typename cx_of<T>::type *__restrict__ psi, double2 *__restrict__ bpsi,
-                               double2 *__restrict__ bdpsi, int n1, int n2, int ncols, int s, double dr,
-                               StencilArg st, int slot) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nx = n1 * ncols;
-  if (x >= nx) return;
-  const int i = x % n1, k = x / n1;
-  const size_t base = (size_t)i + (size_t)n1 * n2 * k;
-  auto F = [&](int j) {
-    const typename cx_of<T>::type v = psi[base + (size_t)n1 * j];
-    return make_double2((double)v.x, (double)v.y);
-  };
-  double2 acc = make_double2(0.0, 0.0);
-  for (int j = st.h; j >= 1; j--) {
-    const double2 a = F(s + j), b = F(s - j);
-    acc.x += st.num[j - 1] * (a.x - b.x);
-    acc.y += st.num[j - 1] * (a.y - b.y);
-  }
-  const double den = st.den * dr;
-  bpsi[(size_t)slot * nx + x] = F(s);
-  bdpsi[(size_t)slot * nx + x] = make_double2(acc.x / den, acc.y / den);
-}
+// (The flux-surface extraction, k_flux_extract, is defined with the r pass's
+// arguments above: the register-FFT r pass can run it in its first work groups.)
 
 // ---------------------------------------------------------------------------
 // Time -> energy accumulation over one time block (CRPAux.F:4-31 summed over

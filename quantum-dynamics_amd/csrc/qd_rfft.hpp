@@ -936,6 +936,14 @@ __global__ __launch_bounds__(NTR) void k_rline_rf(RLineArgs<T> A) {
   TW *s_twa = reinterpret_cast<TW *>(smem_raw);
   TK *s_kin = reinterpret_cast<TK *>(s_twa + N1 * (N2 + 1));
   C *xch = reinterpret_cast<C *>(s_kin + N);  // [N1][N2][LS]
+  // the last ext.nwg work groups extract the previous step's flux surface
+  // (it reads psi, which this pass does not touch) and are done; they are
+  // dispatched as the persistent groups finish, into the pass's tail
+  const int nblk = (int)gridDim.x - A.ext.nwg, bid = (int)blockIdx.x;
+  if (bid >= nblk) {
+    flux_extract_at<T>(A.ext, (bid - nblk) * NTR + (int)threadIdx.x);
+    return;
+  }
   const int t = threadIdx.x;
   const int L = t % LPB, a = t / LPB;
   const rf_pos<N1, N2, sizeof(T) == 4> P(a);
@@ -964,13 +972,13 @@ __global__ __launch_bounds__(NTR) void k_rline_rf(RLineArgs<T> A) {
       if (N % NTR == 0 || q * NTR + t < N) phr[q] = buf_ld<TB>(rp, (unsigned)((q * NTR + t) * sizeof(TB)));
   };
   auto j_of = [&](int b) { return __builtin_amdgcn_readfirstlane(A.fcol_j[(b * LPB) / A.n1]); };
-  int b = blockIdx.x;
+  int b = bid;
   if (b < nbatch) fetch(b, j_of(b), true);
   auto xidx = [L](int n, int k) { return (n * N2 + k) * LS + L; };
   int buf = 0;
-  for (; b < nbatch; b += gridDim.x) {
-    const bool has_n = b + (int)gridDim.x < nbatch;
-    const int bn = has_n ? b + (int)gridDim.x : b;
+  for (; b < nbatch; b += nblk) {
+    const bool has_n = b + nblk < nbatch;
+    const int bn = has_n ? b + nblk : b;
     const int j_n = j_of(bn);  // read while nothing else is outstanding
     TB *ph = s_ph + buf * N;
 #pragma unroll
