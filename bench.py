@@ -54,10 +54,11 @@ B_ALG = {
 }
 # HBM bytes per launch and grid point, measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (fetch scaled x2 by a 1 GiB copy kernel) over the launches of
-# timed steps (those write the flux rows); profiles/*.md.
+# timed steps (the theta step writes the flux rows there, and the r pass also
+# extracts the previous step's flux surface: + 0.63 B); profiles/*.md.
 MEASURED_B_PER_POINT = {
-    "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 16.4},  # batched_step_kernels_FH2_J0-20.md (fused theta)
-    "single": {"theta": 24.65, "fft_R": 16.7, "fft_r": 16.5},  # hbm_traffic_FH2_256x192x120.md (wave-pair theta)
+    "batched": {"theta": 17.2, "fft_R": 16.4, "fft_r": 17.0},  # batched_step_kernels_FH2_J0-20.md (fused theta)
+    "single": {"theta": 24.65, "fft_R": 16.7, "fft_r": 17.2},  # hbm_traffic_FH2_256x192x120.md (wave-pair theta)
 }
 TK_THETA, TK_FFTR1, TK_FFTR2, TK_FLUX, TK_OTHER, TK_ACC = 1, 2, 3, 4, 5, 6
 KCLASSES = (("theta", TK_THETA), ("fft_R", TK_FFTR1), ("fft_r", TK_FFTR2), ("flux", TK_FLUX), ("other", TK_OTHER),
