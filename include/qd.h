@@ -261,7 +261,10 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * wave pairs, bases of 65..128); QD_ERR_ARG when the handle does not have that
  * kernel.
  * what 4: threads per work group of the register-FFT r pass, 256 or 512
- * (batches of twice the consecutive R points); QD_ERR_ARG when it does not fit. */
+ * (batches of twice the consecutive R points); QD_ERR_ARG when it does not fit.
+ * what 5: the fused step's flux-surface extraction run by the next step's
+ * register-FFT r pass (value 1, the default) or as a launch of its own after
+ * each step (0). */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

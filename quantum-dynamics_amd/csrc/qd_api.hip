@@ -189,6 +189,7 @@ struct qd_handle {
   // the next r pass, or runs on its own before anything reads its slot or psi)
   bool ext_pending = false;
   int ext_slot = 0;
+  bool ext_in_rline = true;  // defer the extraction into the r pass (test hook: off)
   // captured flux block (tblock steps + accumulation) for qd_options.use_graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -914,7 +915,7 @@ int step_impl(qd_handle *h, bool diag = false, bool psi_out = false) {
   }
   QD_HIP(hipGetLastError());
   if (h->do_crp) {
-    if (h->fuse_theta && !diag && h->rl_rf) {
+    if (h->fuse_theta && !diag && h->rl_rf && h->ext_in_rline) {
       // deferred to the next r pass (or run on its own by flush_flux,
       // materialize_psi or the end of the call)
       h->ext_pending = true;
@@ -2747,6 +2748,13 @@ int qd_debug_select(qd_handle *h, int32_t what, int32_t value) {
     if (value < 1 || value > 64) return fail(QD_ERR_ARG, "chunk must be 1..64");
     drop_graph(h);
     h->mx_chunk = value;
+    return QD_OK;
+  }
+  if (what == 5) {
+    if (value != 0 && value != 1) return fail(QD_ERR_ARG, "value must be 0 or 1");
+    QD_RC(resolve_ext(h));
+    drop_graph(h);
+    h->ext_in_rline = value == 1;
     return QD_OK;
   }
   if (what == 4) {

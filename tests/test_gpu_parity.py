@@ -205,6 +205,35 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
         assert abs(n_f - n_d) <= 2 * abs(n_u - n_d) + 1e-6 * n_d
 
 
+@pytest.mark.parametrize("variant", ["cpu", "cuda"])
+@pytest.mark.parametrize("kind", ["single", "batched"])
+def test_flux_extraction_in_r_pass_same_bits(qd, kind, variant):
+    """The fused step's flux-surface extraction run by the next step's r pass
+    (the default) against a launch of its own after each step: the same psi
+    and P(E) bit for bit, across a replayed 64-step flux block (graphs on), a
+    P(E) read inside a block, a resume and (single grid) a diagnostics step."""
+    pb = qd.problem
+    print("case: extraction in the r pass,", kind, variant, flush=True)
+    if kind == "single":
+        prob = pb.config(0, steps=100)
+        mk = lambda: qd.Propagator(prob.copy(), precision="mixed", variant=variant, use_graph=1, diagnostics_every=50)
+    else:
+        base = pb.config(0, nR=64, nr=32, ntheta=12, steps=100, e0_ev=0.6)
+        jk = pb.make_jk_problem(base, pb.H_H2, [0, 3, 7], jmax=11, R0=8.0, delta=0.3, e0_ev=0.6)
+        mk = lambda: qd.propagator.JKPropagator(jk, precision="mixed", variant=variant, use_graph=1)
+    out = []
+    for deferred in (1, 0):
+        with mk() as prop:
+            assert prop.tuning()["fused_theta"] >= 1
+            prop.debug_select(5, deferred)
+            prop.propagate(7)
+            c1 = prop.crp().copy()
+            prop.propagate(90)
+            out.append((c1, prop.crp().copy(), prop.psi().copy(), prop.norm()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.fixture(scope="module")
 def wide_r(qd):
     """H+H2 on a finer r grid (192 points over 0.4..6 a0) with the surface at
