@@ -262,7 +262,11 @@ def roofline(kern, npt, workload, fused_theta, prec="mixed"):
             "note": ("the R pass with K<->l mixing is not HBM-bound: about 127 VALU instructions per point (62 the "
                      "register FFT with its phases, 65 the mixing in both directions), VALU issuing on about 58 % of "
                      "its cycles, and waits at the bundle barriers around the mixing; it moves its 16 algorithmic "
-                     "bytes per point (16.4 measured)" if (dom == "fft_R" and workload == "batched") else "")}
+                     "bytes per point (16.4 measured)" if (dom == "fft_R" and workload == "batched") else
+                     "the fused theta step on one 256x192x120 grid (wave pairs, one per component) is latency-bound: "
+                     "256 work groups of eight waves, one per CU (110.6 KB of theta matrices in LDS), waiting on "
+                     "about 46 % of their wave cycles; it moves its 24.5 algorithmic bytes per point (24.65 "
+                     "measured)" if (dom == "theta" and workload == "single") else "")}
 
 
 def main():
