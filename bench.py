@@ -333,8 +333,11 @@ def main():
     s_mixed = timed_single(qd, prob, "mixed", "fused", K, W, autotune=1)
     s_mixed["kernel_ms"] = timed_single(qd, prob, "mixed", "fused", max(20, K // 4), 5, profile=1,
                                         autotune=1)["kernel_ms"]
-    s_f64 = timed_single(qd, prob, "fp64", "fused", max(20, K // 2), 5, profile=1)
-    s_plain = timed_single(qd, prob, "fp64", "plain", max(20, K // 2), 5)
+    # every leg that enters a ratio is timed over the same window of K steps,
+    # with its folds and P(E) inside (a shorter window would charge a leg
+    # more fold per step)
+    s_f64 = timed_single(qd, prob, "fp64", "fused", K, W, profile=1)
+    s_plain = timed_single(qd, prob, "fp64", "plain", K, W)
     single = {
         "workload": "F+H2 J=0 256x192x120", "grid_points": npt1,
         "mixed": {"value": npt1 / (s_mixed["wall_ms_per_step"] * 1e-3), **s_mixed},
@@ -352,7 +355,7 @@ def main():
         m = timed_batched(qd, jk, "mixed", K, W, autotune=1)
         prof = timed_batched(qd, jk, "mixed", max(10, K // 10), 2, profile=1, autotune=1)
         f64 = timed_batched(qd, jk, "fp64", max(10, K // 10), 2, autotune=1)
-        plain_ms, plain_per_n, counts = plain_fp64_same_grids(qd, jk, max(10, K // 10), 3)
+        plain_ms, plain_per_n, counts = plain_fp64_same_grids(qd, jk, K, W)
         step_ms = m["wall_ms_per_step"]
         value = npt / (step_ms * 1e-3)
         plain = {"value": npt / (plain_ms * 1e-3), "ms_per_step": plain_ms,
