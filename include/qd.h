@@ -264,7 +264,10 @@ int qd_get_tuning(qd_handle *h, int32_t *vals);
  * (batches of twice the consecutive R points); QD_ERR_ARG when it does not fit.
  * what 5: the fused step's flux-surface extraction run by the next step's
  * register-FFT r pass (value 1, the default) or as a launch of its own after
- * each step (0). */
+ * each step (0).
+ * what 6: the wave-pair fused theta kernel's two waves of a tile meet through
+ * LDS counters (value 1, the default) or at work-group barriers (0); the same
+ * bits either way.  QD_ERR_ARG on a handle without the wave-pair kernel. */
 int qd_debug_select(qd_handle *h, int32_t what, int32_t value);
 int qd_debug_get_fbr(qd_handle *h, double *out);
 int qd_debug_set_fbr(qd_handle *h, const double *in);

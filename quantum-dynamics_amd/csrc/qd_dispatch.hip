@@ -248,10 +248,11 @@ const void *theta_fused_par_kernel(int ksh, int nth, size_t *lds) {
 
 // fused kernel on the parity halves with each tile shared by a pair of waves
 // (one per component): half bases of 33..64
-const void *theta_fused_pair_kernel(int nth, size_t *lds) {
+const void *theta_fused_pair_kernel(int nth, bool psync, size_t *lds) {
   if (nth != 512) return nullptr;
   *lds = theta_fused_pair_lds<512>();
-  return reinterpret_cast<const void *>(&k_theta_fused_pair<512>);
+  return psync ? reinterpret_cast<const void *>(&k_theta_fused_pair<512, true>)
+               : reinterpret_cast<const void *>(&k_theta_fused_pair<512, false>);
 }
 #endif  // theta part
 

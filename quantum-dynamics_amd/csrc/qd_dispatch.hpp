@@ -41,6 +41,6 @@ const void *theta_mx_kernel(int ks, int pro, int epi);
 // threads per work group (512); *lds = dynamic LDS bytes
 const void *theta_fused_kernel(int ks, int nth, size_t *lds);
 const void *theta_fused_par_kernel(int ksh, int nth, size_t *lds);
-const void *theta_fused_pair_kernel(int nth, size_t *lds);
+const void *theta_fused_pair_kernel(int nth, bool psync, size_t *lds);
 
 }  // namespace qd

@@ -1704,11 +1704,18 @@ __device__ __forceinline__ void buf_st_f(float x, __amdgpu_buffer_rsrc_t r, unsi
 // eight waves (two per SIMD) fit in 256 registers, where one wave per tile
 // needs about 400 and leaves one wave per SIMD.  Loads and stores move one
 // float component (4 bytes at an 8-byte stride; the pair's two halves meet
-// in the caches).  The pair meets at work-group barriers, so every wave runs
-// the same number of (tile, block) iterations; tiles past the item's end are
-// masked (loads read zeros, stores are dropped).  Formulas, middle node,
+// in the caches).  The pair meets twice per round, so both its waves run the
+// same number of (tile, block) iterations (every wave of the group does);
+// tiles past the item's end are masked (loads read zeros, stores are dropped).  Formulas, middle node,
 // dump, flux rows and both propagator orders as in k_theta_fused_par.
-template <int NTH>
+// PSYNC: the two waves of a pair meet through a pair of LDS counters instead
+// of work-group barriers, so the pairs of a work group need not move in step:
+// each wave publishes its count of meetings after its LDS writes (release)
+// and waits until its partner's count has caught up (acquire).  Both waves
+// of a pair make the same meetings, and the wait gives up after about 2^22
+// polls (tenths of a second; a meeting takes microseconds), so it ends even
+// if that did not hold.
+template <int NTH, bool PSYNC>
 __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
   extern __shared__ __align__(16) unsigned char tm_lds[];
   constexpr int KSH = 2, KP = 64, NT = 4, NP = 64, LD = KP + 8, XS = 2 * 32 + 1, NPR = NTH / 128;
@@ -1720,6 +1727,23 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
   // the pair's tile: [16 points][XS]; columns 0..31 DVR rows 32 s2 + j, 32..63 their mirrors
   float2 *Xp = reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + pr * 16 * XS;
   float *Xf = reinterpret_cast<float *>(Xp);
+  // the pairs' meeting counters, after the tiles: [pair][wave of the pair]
+  int *meet = reinterpret_cast<int *>(reinterpret_cast<float2 *>(M + 4 * 3 * NP * LD) + NPR * 16 * XS);
+  int nmeet = 0;
+  auto pair_meet = [&]() {
+    if constexpr (PSYNC) {
+      nmeet++;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __hip_atomic_store(&meet[2 * pr + h], nmeet, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (int poll = 0; poll < (1 << 22); poll++) {
+        if (__hip_atomic_load(&meet[2 * pr + (h ^ 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= nmeet) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+      __syncthreads();
+    }
+  };
   const int4 item = P.items[blockIdx.x];
   const int gb0 = item.x, gnb = item.y, t0 = item.z, t1 = item.w;
   const BlockDesc bd = P.blocks[gb0];  // the group shares mat, nd, nf and the V columns
@@ -1776,6 +1800,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
       *reinterpret_cast<bf8v *>(&M[pn * LD + kq * 8]) = *reinterpret_cast<const bf8v *>(&G[(size_t)pn * KP + kq * 8]);
     }
   }
+  if (threadIdx.x < 2 * NPR) meet[threadIdx.x] = 0;
   __syncthreads();
   for (int it = 0; it < nit; it++) {
     // R as an opaque per-iteration value: the offsets R c 8 are formed where
@@ -1862,7 +1887,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
           Xf[(rl * XS + j) * 2 + h] = c1;
           Xf[(rl * XS + 32 + j) * 2 + h] = c2;
         }
-      __syncthreads();
+      pair_meet();
       // V phase at the B-operand positions, component h; sums and differences
       float sm[8], df[8];
 #pragma unroll
@@ -1874,7 +1899,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
         sm[e] = p1 + p2;
         df[e] = p1 - p2;
       }
-      __syncthreads();  // reads done before the next round's writes
+      pair_meet();  // reads done before the next round's writes
       // forward k-step s2: even FBR columns from the sums, odd from the differences
 #pragma unroll
       for (int q = 0; q < 2; q++) {
@@ -1913,7 +1938,7 @@ __global__ __launch_bounds__(NTH) void k_theta_fused_pair(ThetaFusedArgs P) {
 }
 template <int NTH>
 constexpr size_t theta_fused_pair_lds() {
-  return (size_t)4 * 3 * 64 * 72 * 2 + (size_t)(NTH / 128) * 16 * 65 * 8;
+  return (size_t)4 * 3 * 64 * 72 * 2 + (size_t)(NTH / 128) * 16 * 65 * 8 + (size_t)(NTH / 128) * 2 * sizeof(int);
 }
 
 template <int KS, int NT>

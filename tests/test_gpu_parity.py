@@ -206,6 +206,29 @@ def test_fused_theta_step_matches_unfused(qd, kind, variant):
 
 
 @pytest.mark.parametrize("variant", ["cpu", "cuda"])
+def test_wave_pair_meeting_through_lds_same_bits(qd, variant):
+    """The wave-pair theta kernel with the pair meeting through LDS counters
+    (the default; qd_debug_select what 6) against its work-group-barrier form: the same
+    arithmetic, so psi and P(E) bit for bit, on a 101-point theta grid (half
+    bases of 51, odd middle node) across a replayed 64-step flux block, a
+    P(E) read and a resume."""
+    pb = qd.problem
+    prob = pb.config(0, ntheta=101, steps=80)
+    out = {}
+    for meet in (0, 1):
+        print("case: wave-pair theta, LDS meeting", meet, variant, flush=True)
+        with qd.Propagator(prob.copy(), precision="mixed", variant=variant, use_graph=1) as prop:
+            assert prop.tuning()["fused_theta"] == 3
+            prop.debug_select(6, meet)
+            prop.propagate(70)
+            c1 = prop.crp()
+            prop.propagate(10)
+            out[meet] = (prop.psi(), c1, prop.crp())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("variant", ["cpu", "cuda"])
 @pytest.mark.parametrize("kind", ["single", "batched"])
 def test_flux_extraction_in_r_pass_same_bits(qd, kind, variant):
     """The fused step's flux-surface extraction run by the next step's r pass
