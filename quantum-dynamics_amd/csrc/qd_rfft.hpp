@@ -281,12 +281,25 @@ constexpr bool rf_length(int n) { return rf_n1(n) != 0; }
 // twiddles W_N^(a' k') stay within a quarter turn, which the three-shear
 // rotation needs.  So thread a holds point a' + N1 m in v[m] (rf_pos) and the
 // kinetic table is stored in the threads' order.  FP64 keeps a' = a, k' = k.
-template <typename T, int N1, int N2, typename XI>
+// WAVE: the N1 threads of every line lie in one wave (consecutive threads,
+// N1 dividing 64), so the two exchanges through the line's LDS rows need only
+// the wave's own order, not a work-group barrier.
+template <typename T, int N1, int N2, bool WAVE = false, typename XI>
 __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool act, typename cx_of<T>::type *xb,
                                         XI xidx, const typename rf_tab<T>::tw *twa,
                                         const typename rf_tab<T>::kin *kin) {
   typedef typename cx_of<T>::type C;
   constexpr int XS = N2 + 1;
+  static_assert(!WAVE || 64 % N1 == 0, "a line's threads must stay within one wave");
+  auto exchange_sync = [] {
+    if constexpr (WAVE) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  };
   if (act) {
     rdft<T, N2, true>(v);
 #pragma unroll
@@ -294,7 +307,7 @@ __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool 
 #pragma unroll
     for (int k = 0; k < N2; k++) xb[xidx(a, k)] = v[k];
   }
-  __syncthreads();
+  exchange_sync();
   if (act && a < N2) {
     C u[N1];
 #pragma unroll
@@ -308,7 +321,7 @@ __device__ __forceinline__ void rf_line(typename cx_of<T>::type *v, int a, bool 
 #pragma unroll
     for (int c = 0; c < N1; c++) xb[xidx(c, a)] = u[c];
   }
-  __syncthreads();
+  exchange_sync();
   if (act) {
 #pragma unroll
     for (int k = 0; k < N2; k++) v[k] = xb[xidx(a, k)];
@@ -479,7 +492,8 @@ __global__ __launch_bounds__(LW * N1) void k_rpass_rf(RPassArgs<T> A) {
       // unconditional (empty descriptors past the last batch) keeps the loop
       // body straight-line, so the waits stay exact across iterations
       fetch(bn, lam_n, has_n);
-      rf_line<T, N1, N2>(v, a, true, xb, xidx, s_twa, s_kin);
+      // line L's threads are t = N1 L .. N1 L + N1 - 1: one wave
+      rf_line<T, N1, N2, true>(v, a, true, xb, xidx, s_twa, s_kin);
       C *dst = A.data + (b * LW + L) * N;
 #pragma unroll
       for (int m = 0; m < N2; m++) dst[P(m)] = rf_ph(v[m], ph[P(m)]);
@@ -872,7 +886,8 @@ __global__ __launch_bounds__(NL * N1, TWO ? NL * N1 / 128 : 1) void k_rpass_mx(R
           }
         }
       }
-      rf_line<float, N1, N2>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
+      // line L's threads are t = N1 L .. N1 L + N1 - 1: one wave
+      rf_line<float, N1, N2, true>(v, a, act, row, [](int n, int k) { return n * XS + k; }, s_twa, s_kin);
       if (act) {
         const TB *ph = A.dR + (size_t)lam * N;
 #pragma unroll
